@@ -1,0 +1,447 @@
+// api.hip — the C ABI (include/sjb.h): contexts and lazy window joins.
+//
+// A context owns one HIP stream and every device buffer a window of
+// max_window_tuples per side needs, allocated once at sjb_open (the
+// reference allocates per executor in the constructor, e.g.
+// src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:266-287, and per window in
+// its CountKernel/ES executors).  A window is enqueued with no host sync
+// between its kernels (the reference performs 11 cudaStreamSynchronize
+// calls and a host prefix sum per HJ window,
+// src/sjb/windowing/lazy/executor/GPU/GPU_HJ_NoOutput.cu:43-98).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "sjb_device.hpp"
+#include "sjb_internal.hpp"
+
+namespace sjb {
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    if (getenv("SJB_VERBOSE"))
+        fprintf(stderr, "sjb: %s failed at %s:%d: %s\n", what, file, line, hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory) return SJB_E_NOMEM;
+    return SJB_E_HIP;
+}
+
+int dev_alloc(sjb_ctx *c, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return e == hipErrorOutOfMemory ? SJB_E_NOMEM : SJB_E_HIP;
+    }
+    c->dev_allocs.push_back(*p);
+    c->bytes += bytes;
+    return SJB_OK;
+}
+
+int host_alloc(sjb_ctx *c, void **p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return SJB_E_NOMEM;
+    }
+    c->host_allocs.push_back(*p);
+    return SJB_OK;
+}
+
+bool device_accessible(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.devicePointer != nullptr;
+}
+
+int out_prepare(sjb_ctx *c, sjb_result_tuple_t *out, uint64_t cap, OutBuf *ob) {
+    ob->cap = cap;
+    ob->host = nullptr;
+    ob->dev = out;
+    if (cap == 0 || out == nullptr || device_accessible(out)) return SJB_OK;
+    if (c->out_stage_cap < cap) {
+        void *p = nullptr;
+        int rc = dev_alloc(c, &p, cap * sizeof(sjb_result_tuple_t));
+        if (rc) return rc;
+        c->d_out_stage = static_cast<sjb_result_tuple_t *>(p);
+        c->out_stage_cap = cap;
+    }
+    ob->dev = c->d_out_stage;
+    ob->host = out;
+    return SJB_OK;
+}
+
+int out_finish(sjb_ctx *c, const OutBuf &ob, uint64_t rows) {
+    if (!ob.host || rows == 0) return SJB_OK;
+    SJB_HIP_TRY(hipMemcpyAsync(ob.host, ob.dev, std::min(rows, ob.cap) * sizeof(sjb_result_tuple_t),
+                               hipMemcpyDeviceToHost, c->stream));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SJB_OK;
+}
+
+static bool valid_alg(int alg) { return alg == SJB_ALG_NLJ || alg == SJB_ALG_HJ || alg == SJB_ALG_SMJ; }
+static bool valid_method(int m) { return m >= SJB_OUT_NOOUTPUT && m <= SJB_OUT_ATOMIC; }
+
+// Enqueues one window (count/checksums or pair output) on the ctx stream;
+// the result lands in d_res.  write: 0 = count, 1 = materialise.
+int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
+                          const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
+                          uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
+                          bool timing) {
+    int rc;
+    if (write) {
+        rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
+        if (rc) return rc;
+    }
+    if (nL == 0 || nR == 0) {
+        *logF_used = 0;
+        return launch_fill_u64(c->stream, reinterpret_cast<uint64_t *>(d_res), 0,
+                               sizeof(sjb_window_result_t) / 8);
+    }
+    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (alg == SJB_ALG_NLJ) {
+        *logF_used = 0;
+        if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        rc = launch_nlj(c, L, nL, R, nR, flags, write, out, out_cap, wrap, d_res);
+        if (rc) return rc;
+        if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+        return SJB_OK;
+    }
+    WindowPlan p;
+    p.alg = alg;
+    p.flags = flags;
+    p.nL = nL;
+    p.nR = nR;
+    p.logF = choose_logF(alg, nL, int(c->opts.fanout_log2));
+    p.kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
+             : alg == SJB_ALG_SMJ                     ? PART_RANGE
+                                                      : PART_HASH;
+    p.payload = write ? PAY_IDX : PAY_VAL;
+    p.ncL = int((nL + kChunk - 1) / kChunk);
+    p.ncR = int((nR + kChunk - 1) / kChunk);
+    *logF_used = p.logF;
+    if (p.kind == PART_RANGE) {
+        rc = launch_minmax(c, L, nL, R, nR, p.logF);
+        if (rc) return rc;
+    }
+    rc = launch_partition(c, p, L, R);
+    if (rc) return rc;
+    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
+                           : launch_smj_join(c, p, L, R, out, out_cap, wrap);
+    if (rc) return rc;
+    rc = launch_reduce(c, 1 << p.logF, d_res);
+    if (rc) return rc;
+    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+    return SJB_OK;
+}
+
+static int fetch_result(sjb_ctx *c, sjb_window_result_t *host) {
+    SJB_HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(sjb_window_result_t),
+                               hipMemcpyDeviceToHost, c->stream));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    *host = *c->h_result;
+    return SJB_OK;
+}
+
+static void fill_times(sjb_ctx *c, int alg, bool timing, sjb_stats_t *st) {
+    if (!timing) return;
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, c->ev0, c->ev1) != hipSuccess) a = 0.f;
+    if (hipEventElapsedTime(&b, c->ev1, c->ev2) != hipSuccess) b = 0.f;
+    const uint64_t ua = uint64_t(a * 1000.f), ub = uint64_t(b * 1000.f);
+    st->device_ns += uint64_t((a + b) * 1.0e6f);
+    if (alg == SJB_ALG_SMJ) {
+        st->sort_time_us += ua;
+        st->merge_time_us += ub;
+    } else {
+        st->build_time_us += ua;
+        st->probe_time_us += ub;
+    }
+}
+
+// Core of sjb_lazy_join for device-resident inputs.
+static int lazy_join_device(sjb_ctx *c, int alg, int method, uint32_t flags, const sjb_tuple_t *L,
+                            uint64_t nL, const sjb_tuple_t *R, uint64_t nR, sjb_result_tuple_t *out,
+                            uint64_t out_cap, sjb_stats_t *st) {
+    flags |= c->opts.flags;
+    const bool timing = (flags & SJB_F_TIMING) != 0;
+    sjb_window_result_t res{};
+    OutBuf ob;
+    if (method != SJB_OUT_NOOUTPUT) {
+        if (out_cap > 0 && out == nullptr) return SJB_E_INVALID;
+        int r0 = out_prepare(c, out, out_cap, &ob);
+        if (r0) return r0;
+        out = ob.dev;
+    }
+    int logF = 0, rc;
+    const bool c2_enum = (flags & SJB_F_C2) != 0 && !(flags & SJB_F_GROUPED);
+    if (method == SJB_OUT_BITMAP) {
+        if (alg != SJB_ALG_NLJ) return SJB_E_UNSUPPORTED;
+        uint64_t nm = 0;
+        rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
+        if (rc) return rc;
+        if (nL && nR) {
+            if (timing) SJB_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+            if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+            rc = launch_nlj_bitmap(c, L, nL, R, nR, flags, out, out_cap, c->d_result, &nm);
+            if (rc) return rc;
+            if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+            rc = fetch_result(c, &res);
+            if (rc) return rc;
+        }
+        st->result_count = res.count;
+        st->checksum1 = res.checksum1;
+        st->checksum2 = res.checksum2;
+        st->c2_valid = 1;
+        st->pairs_written = std::min(res.count, out_cap);
+        fill_times(c, alg, timing && nL && nR, st);
+        rc = out_finish(c, ob, st->pairs_written);
+        if (rc) return rc;
+        return res.count > out_cap ? SJB_E_CAPACITY : SJB_OK;
+    }
+    if (method == SJB_OUT_NOOUTPUT) {
+        rc = window_enqueue(c, alg, flags, L, nL, R, nR, 0, nullptr, 0, 0, c->d_result, &logF, timing);
+        if (rc) return rc;
+        rc = fetch_result(c, &res);
+        if (rc) return rc;
+        st->c2_valid = c2_enum ? 1 : 0;
+    } else {
+        if (out_cap > 0 && out == nullptr) return SJB_E_INVALID;
+        if (method == SJB_OUT_COUNTKERNEL) {
+            // pass 1: exact count (GPU_HJ_CountKernel.cu:83-101 counts first)
+            rc = window_enqueue(c, alg, flags & ~SJB_F_C2, L, nL, R, nR, 0, nullptr, 0, 0,
+                                c->d_result, &logF, false);
+            if (rc) return rc;
+            rc = fetch_result(c, &res);
+            if (rc) return rc;
+            if (res.count > out_cap) {
+                st->result_count = res.count;
+                st->checksum1 = res.checksum1;
+                st->fanout_log2 = uint32_t(logF);
+                return SJB_E_CAPACITY;
+            }
+        }
+        if (method == SJB_OUT_ATOMIC && out_cap == 0) return SJB_E_INVALID;
+        const int wrap = method == SJB_OUT_ATOMIC ? 1 : 0;
+        rc = window_enqueue(c, alg, flags | SJB_F_C2, L, nL, R, nR, 1, out, out_cap, wrap,
+                            c->d_result, &logF, timing);
+        if (rc) return rc;
+        rc = fetch_result(c, &res);
+        if (rc) return rc;
+        st->c2_valid = 1;
+        st->pairs_written = std::min(res.count, out_cap);
+        rc = out_finish(c, ob, st->pairs_written);
+        if (rc) return rc;
+        if (method != SJB_OUT_ATOMIC && res.count > out_cap) rc = SJB_E_CAPACITY;
+    }
+    st->result_count = res.count;
+    st->checksum1 = res.checksum1;
+    st->checksum2 = st->c2_valid ? res.checksum2 : 0;
+    st->fanout_log2 = uint32_t(logF);
+    fill_times(c, alg, timing && nL && nR, st);
+    return rc;
+}
+
+}  // namespace sjb
+
+using namespace sjb;
+
+extern "C" {
+
+int sjb_abi_version(void) { return SJB_ABI_VERSION; }
+
+const char *sjb_strerror(int code) {
+    switch (code) {
+        case SJB_OK: return "ok";
+        case SJB_E_INVALID: return "invalid argument";
+        case SJB_E_CAPACITY: return "capacity exceeded";
+        case SJB_E_NOMEM: return "out of memory";
+        case SJB_E_HIP: return "HIP runtime error";
+        case SJB_E_UNSUPPORTED: return "unsupported combination";
+        case SJB_E_NODEVICE: return "no usable device";
+        case SJB_E_BUDGET: return "HBM budget exceeded";
+        case SJB_E_TIMEOUT: return "bounded wait expired";
+        case SJB_E_STATE: return "invalid state";
+    }
+    return "unknown error";
+}
+
+int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
+    if (!out) return SJB_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return SJB_E_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return SJB_E_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SJB_E_NODEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SJB_E_NODEVICE;
+    SJB_HIP_TRY(hipSetDevice(device));
+
+    sjb_ctx *c = new sjb_ctx();
+    c->device = device;
+    if (opts) c->opts = *opts;
+    if (c->opts.max_window_tuples == 0) c->opts.max_window_tuples = 1ull << 22;
+    if (c->opts.num_distinct_keys == 0) c->opts.num_distinct_keys = 1000;
+    if (c->opts.batch_size == 0) c->opts.batch_size = 8192;
+    c->n_cu = prop.multiProcessorCount;
+    c->cap = c->opts.max_window_tuples;
+    int rc = SJB_OK;
+    auto fail = [&](int r) {
+        sjb_close(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SJB_E_HIP);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess)
+        return fail(SJB_E_HIP);
+
+    const uint64_t cap = c->cap;
+    if (c->opts.hbm_budget_bytes) {
+        const uint64_t need = cap * (2 * sizeof(sjb_tuple_t) + 2 * sizeof(Rec));
+        if (need > c->opts.hbm_budget_bytes) return fail(SJB_E_BUDGET);
+    }
+    const uint64_t nchunks = (cap + kChunk - 1) / kChunk;
+    c->hist_cap = (uint64_t(1) << kMaxLogF) * 2 * std::max<uint64_t>(nchunks, 1);
+    const uint64_t scan_blocks = c->hist_cap / 4096 + 2048;
+    const uint64_t n_partials = std::max<uint64_t>(65536, 1ull << kMaxLogF);
+    c->mask_cap = std::max<uint64_t>(cap, 1ull << 20);
+    if ((rc = dev_alloc(c, (void **)&c->d_stage[0], cap * sizeof(sjb_tuple_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_stage[1], cap * sizeof(sjb_tuple_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_part, 2 * cap * sizeof(Rec))) ||
+        (rc = dev_alloc(c, (void **)&c->d_hist, c->hist_cap * sizeof(uint32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_scan_blocks, scan_blocks * sizeof(uint32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_partials, 3 * n_partials * sizeof(uint64_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_result, 64 * sizeof(sjb_window_result_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_pparams, sizeof(PartParams))) ||
+        (rc = dev_alloc(c, (void **)&c->d_rmi, sizeof(RmiParams))) ||
+        (rc = dev_alloc(c, (void **)&c->d_cursor, 4 * sizeof(uint64_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_masks, 3 * c->mask_cap * sizeof(uint64_t))) ||
+        (rc = host_alloc(c, (void **)&c->h_result, 64 * sizeof(sjb_window_result_t))))
+        return fail(rc);
+    if (launch_fill_u64(c->stream, c->d_cursor, 0, 4) != SJB_OK) return fail(SJB_E_HIP);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SJB_E_HIP);
+    *out = c;
+    return SJB_OK;
+}
+
+int sjb_close(sjb_ctx_t *c) {
+    if (!c) return SJB_E_INVALID;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->dev_allocs) (void)hipFree(p);
+    for (void *p : c->host_allocs) (void)hipHostFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SJB_OK;
+}
+
+void *sjb_stream(sjb_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
+
+int sjb_sync(sjb_ctx_t *c) {
+    if (!c) return SJB_E_INVALID;
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SJB_OK;
+}
+
+uint64_t sjb_device_bytes(sjb_ctx_t *c) { return c ? c->bytes : 0; }
+
+int sjb_lazy_join(sjb_ctx_t *c, int algorithm, int method, uint32_t flags, const sjb_tuple_t *d_left,
+                  uint64_t n_left, const sjb_tuple_t *d_right, uint64_t n_right,
+                  sjb_result_tuple_t *out, uint64_t out_cap, sjb_stats_t *stats) {
+    if (!c || !stats || !valid_alg(algorithm) || !valid_method(method)) return SJB_E_INVALID;
+    if ((n_left && !d_left) || (n_right && !d_right)) return SJB_E_INVALID;
+    memset(stats, 0, sizeof(*stats));
+    if (n_left > c->cap || n_right > c->cap) return SJB_E_CAPACITY;
+    SJB_HIP_TRY(hipSetDevice(c->device));
+    if ((n_left && !device_accessible(d_left)) || (n_right && !device_accessible(d_right)))
+        return SJB_E_INVALID;
+    return lazy_join_device(c, algorithm, method, flags, d_left, n_left, d_right, n_right, out,
+                            out_cap, stats);
+}
+
+// Gathers ring slots into the ctx staging buffers.  Adjacent slots are
+// merged into one copy (the reference copies one contiguous run from
+// min(indexes) and is wrong when slots are not adjacent,
+// src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:235-241).
+static int gather(sjb_ctx *c, sjb_tuple_t *dst, const sjb_tuple_t *ring, const uint64_t *idx,
+                  uint64_t n_idx, uint64_t B) {
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < n_idx;) {
+        uint64_t j = i + 1;
+        while (j < n_idx && idx[j] == idx[j - 1] + B) j++;
+        const uint64_t cnt = (j - i) * B;
+        SJB_HIP_TRY(hipMemcpyAsync(dst + w, ring + idx[i], cnt * sizeof(sjb_tuple_t),
+                                   hipMemcpyHostToDevice, c->stream));
+        w += cnt;
+        i = j;
+    }
+    return SJB_OK;
+}
+
+int sjb_lazy_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
+                     const sjb_tuple_t *left_ring, const uint64_t *left_idx, uint64_t n_left_idx,
+                     const sjb_tuple_t *right_ring, const uint64_t *right_idx, uint64_t n_right_idx,
+                     uint64_t batch_size, sjb_result_tuple_t *out, uint64_t out_cap,
+                     sjb_stats_t *stats) {
+    if (!c || !stats || !valid_alg(algorithm) || !valid_method(method) || batch_size == 0)
+        return SJB_E_INVALID;
+    if ((n_left_idx && (!left_ring || !left_idx)) || (n_right_idx && (!right_ring || !right_idx)))
+        return SJB_E_INVALID;
+    memset(stats, 0, sizeof(*stats));
+    const uint64_t nL = n_left_idx * batch_size, nR = n_right_idx * batch_size;
+    if (nL > c->cap || nR > c->cap) return SJB_E_CAPACITY;
+    SJB_HIP_TRY(hipSetDevice(c->device));
+    int rc = gather(c, c->d_stage[0], left_ring, left_idx, n_left_idx, batch_size);
+    if (rc) return rc;
+    rc = gather(c, c->d_stage[1], right_ring, right_idx, n_right_idx, batch_size);
+    if (rc) return rc;
+    return lazy_join_device(c, algorithm, method, flags, c->d_stage[0], nL, c->d_stage[1], nR, out,
+                            out_cap, stats);
+}
+
+int sjb_lazy_join_async(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_tuple_t *d_left,
+                        uint64_t n_left, const sjb_tuple_t *d_right, uint64_t n_right,
+                        sjb_window_result_t *d_result) {
+    if (!c || !d_result || !valid_alg(algorithm)) return SJB_E_INVALID;
+    if ((n_left && !d_left) || (n_right && !d_right)) return SJB_E_INVALID;
+    if (n_left > c->cap || n_right > c->cap) return SJB_E_CAPACITY;
+    if ((n_left && !device_accessible(d_left)) || (n_right && !device_accessible(d_right)) ||
+        !device_accessible(d_result))
+        return SJB_E_INVALID;
+    int logF;
+    return window_enqueue(c, algorithm, flags | c->opts.flags, d_left, n_left, d_right, n_right, 0,
+                          nullptr, 0, 0, d_result, &logF, false);
+}
+
+int sjb_lazy_join_batch(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_tuple_t *d_left,
+                        const uint64_t *left_off, const sjb_tuple_t *d_right,
+                        const uint64_t *right_off, uint32_t n_windows, sjb_window_result_t *results) {
+    if (!c || !left_off || !right_off || !results || !valid_alg(algorithm))
+        return SJB_E_INVALID;
+    return batch_join(c, algorithm, flags | c->opts.flags, d_left, left_off, d_right, right_off,
+                      n_windows, results);
+}
+
+uint64_t sjb_mix64(uint64_t z) { return mix64(z); }
+uint64_t sjb_hash_a(uint32_t k, uint32_t v) { return hash_a(k, v); }
+uint64_t sjb_hash_b(uint32_t k, uint32_t v) { return hash_b(k, v); }
+uint64_t sjb_pair_hash(uint32_t k, uint32_t rv, uint32_t sv) {
+    return pair_hash_from_a(hash_a(k, rv), sv);
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+// nlj.hip — nested-loop join kernels (lazy and eager NLJ).
+//
+// Replaces nestedLoopCountKernel / nestedLoopJoinKernel
+// (src/sjb/windowing/lazy/executor/GPU/GPU_NLJ.cu:28-64), the eager
+// countKernel (src/sjb/windowing/eager/executor/GPU/GPU_NLJ.cu:40-54) and the
+// bitmap kernels getJoinResultBitmap / getJoinResultBitmapEager
+// (GPU_NLJ.cu:66-87, eager/executor/GPU/GPU_NLJ_BM.cu:55-76).
+//
+// Count kernel: each thread keeps 4 right (probe) tuples in registers; the
+// workgroup streams a range of left keys through LDS in 2048-key tiles and
+// every lane compares its 4 keys with every tile key (broadcast LDS reads,
+// 4 keys per ds_read_b128).  The compare is VALU work: x = lk ^ sk and
+// m = min3(m, x0, x1) — 1.5 ops per compare — so a tile with no match costs
+// nothing else.  Only lanes whose m reached 0 rescan the tile exactly and
+// add count, C1 and C2 per matching pair.  The NLJ is VALU-bound (n_L*n_R
+// compares), not HBM-bound.
+//
+// Bitmap method: one wave holds 64 consecutive right keys; for each left
+// key, __ballot(lk == rk) is exactly the 64-bit match word of that left
+// tuple (the reference builds 32-bit words with one atomicAdd each and
+// decodes them on the host).  Non-zero words are appended as (l, w, mask)
+// triples and decoded on the device into ResultTuple rows.
+#include "sjb_device.hpp"
+#include "sjb_internal.hpp"
+
+namespace sjb {
+
+constexpr int kNljThreads = 256;
+constexpr int kNljPer = 4;
+constexpr int kNljRTile = kNljThreads * kNljPer;  // right tuples per workgroup
+constexpr int kNljLTile = 2048;                   // left keys per LDS tile
+constexpr int kNljMaxBlocks = 65536;
+
+template <bool C2, bool WRITE>
+__global__ __launch_bounds__(kNljThreads) void k_nlj(
+    const sjb_tuple_t *__restrict__ L, uint64_t nL, const sjb_tuple_t *__restrict__ R, uint64_t nR,
+    uint64_t l_split, uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out,
+    uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
+    __shared__ __align__(16) uint32_t s_lk[kNljLTile];
+    __shared__ uint64_t s_red[3 * kNljThreads / kWave];
+    __shared__ uint32_t s_ws[kNljThreads / kWave];
+    __shared__ unsigned long long s_base;
+
+    const uint64_t rbase = uint64_t(blockIdx.x) * kNljRTile;
+    uint32_t sk[kNljPer], sv[kNljPer];
+    bool valid[kNljPer];
+#pragma unroll
+    for (int i = 0; i < kNljPer; i++) {
+        const uint64_t r = rbase + threadIdx.x + uint64_t(i) * kNljThreads;
+        valid[i] = r < nR;
+        const uint2 kv = valid[i] ? *reinterpret_cast<const uint2 *>(&R[r]) : make_uint2(0, 0);
+        sk[i] = kv.x;
+        sv[i] = kv.y;
+    }
+    Acc acc;
+    acc.zero();
+    const uint64_t lbeg = uint64_t(blockIdx.y) * l_split;
+    const uint64_t lend = min(nL, lbeg + l_split);
+
+    for (uint64_t t0 = lbeg; t0 < lend; t0 += kNljLTile) {
+        const int tn = int(min<uint64_t>(kNljLTile, lend - t0));
+        for (int i = threadIdx.x; i < tn; i += kNljThreads) s_lk[i] = L[t0 + i].key;
+        __syncthreads();
+        uint32_t m = 0xFFFFFFFFu;
+        const int t4 = tn & ~3;
+        const uint4 *lk4 = reinterpret_cast<const uint4 *>(s_lk);
+        for (int e = 0; e < t4; e += 4) {
+            const uint4 q = lk4[e >> 2];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t lk = c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
+                m = min(m, min(lk ^ sk[0], lk ^ sk[1]));
+                m = min(m, min(lk ^ sk[2], lk ^ sk[3]));
+            }
+        }
+        for (int e = t4; e < tn; e++) {
+            const uint32_t lk = s_lk[e];
+#pragma unroll
+            for (int i = 0; i < kNljPer; i++) m = min(m, lk ^ sk[i]);
+        }
+        // Exact pass only for lanes that saw a zero xor.
+        uint32_t mine = 0;
+        if (m == 0) {
+            for (int e = 0; e < tn; e++) {
+                const uint32_t lk = s_lk[e];
+#pragma unroll
+                for (int i = 0; i < kNljPer; i++) {
+                    if (valid[i] && lk == sk[i]) {
+                        mine++;
+                        if (!WRITE) {
+                            const uint64_t a = hash_a(lk, L[t0 + e].val);
+                            acc.count++;
+                            acc.c1 += a * hash_b(lk, sv[i]);
+                            if (C2) acc.c2 += pair_hash_from_a(a, sv[i]);
+                        }
+                    }
+                }
+            }
+        }
+        if (WRITE) {
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_u32(mine, s_ws, &tot);
+            if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+            __syncthreads();
+            if (mine) {
+                uint64_t pos = s_base + ex;
+                for (int e = 0; e < tn; e++) {
+                    const uint32_t lk = s_lk[e];
+#pragma unroll
+                    for (int i = 0; i < kNljPer; i++) {
+                        if (valid[i] && lk == sk[i]) {
+                            const sjb_tuple_t lt = L[t0 + e];
+                            const sjb_tuple_t st = R[rbase + threadIdx.x + uint64_t(i) * kNljThreads];
+                            const uint64_t a = hash_a(lt.key, lt.val);
+                            acc.count++;
+                            acc.c1 += a * hash_b(st.key, st.val);
+                            if (C2) acc.c2 += pair_hash_from_a(a, st.val);
+                            uint64_t q = pos++;
+                            if (wrap) q %= out_cap;
+                            if (wrap || q < out_cap) {
+                                sjb_result_tuple_t row;
+                                row.lkey = lt.key;
+                                row.rkey = st.key;
+                                row.lval = lt.val;
+                                row.rval = st.val;
+                                row.lts = lt.ts;
+                                row.rts = st.ts;
+                                out[q] = row;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const Acc r = block_sum(acc, s_red);
+    if (threadIdx.x == 0) {
+        const uint64_t b = uint64_t(blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[3 * b] = r.count;
+        partials[3 * b + 1] = r.c1;
+        partials[3 * b + 2] = r.c2;
+    }
+}
+
+int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
+               uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
+               sjb_window_result_t *d_out) {
+    const uint64_t gx = (nR + kNljRTile - 1) / kNljRTile;
+    // Split the left side so the grid fills the chip (>= 4 workgroups per CU)
+    // without exceeding the partials buffer.
+    uint64_t want = uint64_t(c->n_cu) * 4;
+    uint64_t gy = (want + gx - 1) / gx;
+    uint64_t max_gy = (nL + kNljLTile - 1) / kNljLTile;
+    if (gy > max_gy) gy = max_gy;
+    if (gy < 1) gy = 1;
+    if (gx * gy > kNljMaxBlocks) gy = kNljMaxBlocks / gx;
+    if (gy < 1 || gx > 65535) return SJB_E_CAPACITY;
+    uint64_t l_split = (nL + gy - 1) / gy;
+    l_split = (l_split + kNljLTile - 1) / kNljLTile * kNljLTile;
+    gy = (nL + l_split - 1) / l_split;
+    const dim3 grid((unsigned)gx, (unsigned)gy);
+    const bool c2 = (flags & SJB_F_C2) != 0;
+    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+    if (write) {
+        if (c2)
+            k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+                                                                   out, out_cap, wrap, cur);
+        else
+            k_nlj<false, true><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+                                                                    out, out_cap, wrap, cur);
+    } else {
+        if (c2)
+            k_nlj<true, false><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+                                                                    out, out_cap, wrap, cur);
+        else
+            k_nlj<false, false><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split,
+                                                                     c->d_partials, out, out_cap, wrap, cur);
+    }
+    SJB_HIP_TRY(hipGetLastError());
+    return launch_reduce(c, int(gx * gy), d_out);
+}
+
+// ---------------------------------------------------------------------------
+// Bitmap method
+// ---------------------------------------------------------------------------
+constexpr int kBmThreads = 256;  // 4 waves = 4 right words of 64 keys
+constexpr int kBmLTile = 2048;
+
+__global__ __launch_bounds__(kBmThreads) void k_nlj_bitmap(
+    const sjb_tuple_t *__restrict__ L, uint64_t nL, const sjb_tuple_t *__restrict__ R, uint64_t nR,
+    uint64_t l_split, uint64_t *__restrict__ masks, uint64_t mask_cap,
+    unsigned long long *__restrict__ n_masks) {
+    __shared__ uint32_t s_lk[kBmLTile];
+    const uint64_t r = uint64_t(blockIdx.x) * kBmThreads + threadIdx.x;
+    const uint64_t word = r / kWave;
+    const bool valid = r < nR;
+    const uint32_t rk = valid ? R[r].key : 0u;
+    const uint64_t lbeg = uint64_t(blockIdx.y) * l_split;
+    const uint64_t lend = min(nL, lbeg + l_split);
+    for (uint64_t t0 = lbeg; t0 < lend; t0 += kBmLTile) {
+        const int tn = int(min<uint64_t>(kBmLTile, lend - t0));
+        for (int i = threadIdx.x; i < tn; i += kBmThreads) s_lk[i] = L[t0 + i].key;
+        __syncthreads();
+        for (int e = 0; e < tn; e++) {
+            const uint64_t mask = __ballot(valid && s_lk[e] == rk);
+            if (mask != 0 && (threadIdx.x & (kWave - 1)) == 0) {
+                const unsigned long long slot = atomicAdd(n_masks, 1ull);
+                if (slot < mask_cap) {
+                    masks[3 * slot] = t0 + e;
+                    masks[3 * slot + 1] = word;
+                    masks[3 * slot + 2] = mask;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One thread per mask triple: reserve popcount(mask) rows per block, write
+// them, and accumulate count/C1/C2.
+__global__ __launch_bounds__(256) void k_bitmap_decode(
+    const uint64_t *__restrict__ masks, const unsigned long long *__restrict__ n_masks,
+    const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
+    sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, unsigned long long *__restrict__ cursor,
+    uint64_t *__restrict__ partials) {
+    __shared__ uint64_t s_red[3 * 256 / kWave];
+    __shared__ uint32_t s_ws[256 / kWave];
+    __shared__ unsigned long long s_base;
+    const uint64_t n = *n_masks;
+    Acc acc;
+    acc.zero();
+    for (uint64_t i0 = uint64_t(blockIdx.x) * 256; i0 < n; i0 += uint64_t(gridDim.x) * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        uint64_t mask = 0, l = 0, w = 0;
+        if (i < n) {
+            l = masks[3 * i];
+            w = masks[3 * i + 1];
+            mask = masks[3 * i + 2];
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_u32(uint32_t(__popcll(mask)), s_ws, &tot);
+        if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        uint64_t pos = s_base + ex;
+        if (mask) {
+            const sjb_tuple_t lt = L[l];
+            const uint64_t a = hash_a(lt.key, lt.val);
+            while (mask) {
+                const int bit = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const sjb_tuple_t st = R[w * kWave + bit];
+                acc.count++;
+                acc.c1 += a * hash_b(st.key, st.val);
+                acc.c2 += pair_hash_from_a(a, st.val);
+                if (pos < out_cap) {
+                    sjb_result_tuple_t row;
+                    row.lkey = lt.key;
+                    row.rkey = st.key;
+                    row.lval = lt.val;
+                    row.rval = st.val;
+                    row.lts = lt.ts;
+                    row.rts = st.ts;
+                    out[pos] = row;
+                }
+                pos++;
+            }
+        }
+        __syncthreads();
+    }
+    const Acc r = block_sum(acc, s_red);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = r.count;
+        partials[3 * blockIdx.x + 1] = r.c1;
+        partials[3 * blockIdx.x + 2] = r.c2;
+    }
+}
+
+int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
+                      uint64_t nR, uint32_t flags, sjb_result_tuple_t *out, uint64_t out_cap,
+                      sjb_window_result_t *d_out, uint64_t *n_masks_host) {
+    (void)flags;
+    const uint64_t gx = (nR + kBmThreads - 1) / kBmThreads;
+    uint64_t gy = (uint64_t(c->n_cu) * 4 + gx - 1) / gx;
+    const uint64_t max_gy = (nL + kBmLTile - 1) / kBmLTile;
+    if (gy > max_gy) gy = max_gy;
+    if (gy < 1) gy = 1;
+    if (gx > 65535) return SJB_E_CAPACITY;
+    uint64_t l_split = (nL + gy - 1) / gy;
+    l_split = (l_split + kBmLTile - 1) / kBmLTile * kBmLTile;
+    gy = (nL + l_split - 1) / l_split;
+    auto *cnt = reinterpret_cast<unsigned long long *>(c->d_cursor) + 1;
+    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+    int rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
+    if (rc) return rc;
+    k_nlj_bitmap<<<dim3(unsigned(gx), unsigned(gy)), kBmThreads, 0, c->stream>>>(
+        L, nL, R, nR, l_split, c->d_masks, c->mask_cap, cnt);
+    SJB_HIP_TRY(hipGetLastError());
+    unsigned long long nm = 0;
+    SJB_HIP_TRY(hipMemcpyAsync(&nm, cnt, sizeof(nm), hipMemcpyDeviceToHost, c->stream));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    *n_masks_host = nm;
+    if (nm > c->mask_cap) return SJB_E_CAPACITY;
+    const int blocks = 512;
+    k_bitmap_decode<<<blocks, 256, 0, c->stream>>>(c->d_masks, cnt, L, R, out, out_cap, cur,
+                                                   c->d_partials);
+    SJB_HIP_TRY(hipGetLastError());
+    return launch_reduce(c, blocks, d_out);
+}
+
+}  // namespace sjb

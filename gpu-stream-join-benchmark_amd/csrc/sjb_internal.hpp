@@ -1,0 +1,170 @@
+// sjb_internal.hpp — host-side internals shared by the translation units
+// of libsjb_hip.so: the context object, launch helpers and the kernel
+// launcher declarations.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "sjb.h"
+
+#define SJB_HIP_TRY(expr)                                   \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return sjb::hip_fail(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+namespace sjb {
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// Partition record: key plus a 32-bit payload (the tuple's val for
+// count/checksum work, or its index in the window for pair output).
+typedef uint2 Rec;
+
+constexpr int kChunk = 16384;        // tuples per partition-pass workgroup
+constexpr int kPartThreads = 512;    // threads of the partition passes
+constexpr int kMaxLogF = 13;         // fanout bound: 8192 partitions
+constexpr int kJoinThreads = 256;    // threads of the per-partition join
+constexpr int kHjCap = 4096;         // build entries per LDS tile (HJ)
+constexpr int kSmjCap = 2048;        // elements per LDS sort tile (SMJ)
+constexpr int kMaxHeavy = 32;        // heavy-key slots (grouped mode)
+
+enum PartKind : int { PART_HASH = 0, PART_RANGE = 1, PART_LEARNED = 2 };
+enum Payload : int { PAY_VAL = 0, PAY_IDX = 1 };
+
+// Learned CDF model (two-stage RMI): root linear over the key gives a leaf,
+// each leaf is a linear map key -> CDF position in [0,1).
+constexpr int kRmiLeaves = 1024;
+struct RmiParams {
+    float root_a, root_b;          // leaf = clamp(root_a * key + root_b)
+    float leaf_a[kRmiLeaves];      // cdf  = leaf_a * key + leaf_b  (key as float)
+    float leaf_b[kRmiLeaves];
+};
+
+// Device parameters of a partition function (written by a kernel or H2D).
+struct PartParams {
+    uint32_t kmin;        // range partitioning: min key of both sides
+    uint32_t shift;       // range partitioning: key bits dropped
+    uint32_t n_heavy;     // grouped mode: number of heavy keys
+    uint32_t pad;
+    uint32_t heavy[kMaxHeavy];
+};
+
+struct WindowPlan {
+    int alg;
+    int logF;
+    int kind;      // PartKind
+    int payload;   // Payload
+    uint32_t flags;
+    uint64_t nL, nR;
+    int ncL, ncR;  // chunks per side
+};
+
+}  // namespace sjb
+
+struct sjb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sjb_options_t opts{};
+    int n_cu = 0;
+    uint64_t bytes = 0;
+    std::vector<void *> dev_allocs;
+    std::vector<void *> host_allocs;
+
+    // lazy window buffers
+    uint64_t cap = 0;                 // per-side tuple capacity
+    sjb_tuple_t *d_stage[2] = {};     // H2D gather target of sjb_lazy_execute
+    sjb::Rec *d_part = nullptr;       // partitioned records, left then right
+    uint32_t *d_hist = nullptr;       // F * (ncL + ncR) counters, scanned in place
+    uint32_t *d_scan_blocks = nullptr;
+    uint64_t hist_cap = 0;
+    uint64_t *d_partials = nullptr;   // 4 u64 per partition
+    sjb_window_result_t *d_result = nullptr;
+    sjb_window_result_t *h_result = nullptr;  // pinned
+    sjb::PartParams *d_pparams = nullptr;
+    sjb::RmiParams *d_rmi = nullptr;
+    bool rmi_valid = false;
+    uint64_t *d_cursor = nullptr;     // pair-output cursor
+    uint64_t *d_heavy = nullptr;      // heavy-key aggregates
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+
+    // device staging of pair output when the caller's buffer is pageable
+    sjb_result_tuple_t *d_out_stage = nullptr;
+    uint64_t out_stage_cap = 0;
+
+    // batched windows
+    sjb_window_result_t *d_batch_res = nullptr;
+    uint64_t batch_res_cap = 0;
+
+    // NLJ bitmap path
+    uint64_t *d_masks = nullptr;      // (left idx, word idx, mask) triples
+    uint64_t mask_cap = 0;
+
+    // eager state
+    bool eager_ready = false;
+    uint64_t eager_cap = 0;           // tuples per side
+    uint64_t eager_nb = 0;            // buckets per side (pow2)
+    sjb_tuple_t *d_eager_tuples[2] = {};
+    uint32_t *d_eager_next[2] = {};
+    uint32_t *d_eager_head[2] = {};
+    uint64_t eager_n[2] = {0, 0};
+    sjb_tuple_t *d_batch = nullptr;   // staged incoming batch
+    uint64_t batch_cap = 0;
+};
+
+namespace sjb {
+
+// allocation helpers (tracked; freed by sjb_close)
+int dev_alloc(sjb_ctx *c, void **p, size_t bytes);
+int host_alloc(sjb_ctx *c, void **p, size_t bytes);
+
+// ---- launchers (partition.hip) ----
+int choose_logF(int alg, uint64_t n_build, int requested);
+int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
+                  int logF);
+int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
+int launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, uint64_t n);
+int launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, uint64_t n);
+
+// ---- launchers (join.hip) ----
+int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
+                   sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
+int launch_smj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
+                    sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
+int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out);
+
+// ---- launchers (nlj.hip) ----
+int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
+               uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
+               sjb_window_result_t *d_out);
+int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
+                      uint64_t nR, uint32_t flags, sjb_result_tuple_t *out, uint64_t out_cap,
+                      sjb_window_result_t *d_out, uint64_t *n_masks);
+
+// ---- api.hip ----
+int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
+                   const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
+                   uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
+                   bool timing);
+
+// ---- batch.hip ----
+int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const uint64_t *loff,
+               const sjb_tuple_t *R, const uint64_t *roff, uint32_t n_windows,
+               sjb_window_result_t *results);
+
+// ---- pointer checks / output staging (api.hip) ----
+// True when the device can dereference p (device memory, or pinned /
+// registered host memory).
+bool device_accessible(const void *p);
+struct OutBuf {
+    sjb_result_tuple_t *dev = nullptr;   // what kernels write
+    sjb_result_tuple_t *host = nullptr;  // caller's pageable buffer (copy back) or null
+    uint64_t cap = 0;
+};
+int out_prepare(sjb_ctx *c, sjb_result_tuple_t *out, uint64_t cap, OutBuf *ob);
+int out_finish(sjb_ctx *c, const OutBuf &ob, uint64_t rows);
+
+}  // namespace sjb

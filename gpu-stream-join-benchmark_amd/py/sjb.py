@@ -1,0 +1,196 @@
+"""ctypes binding of libsjb_hip.so (the C ABI declared in include/sjb.h).
+
+Python callers (tests, bench.py, __graft_entry__) reach the HIP operators only
+through this module.  It never falls back to anything: if the shared library
+is missing or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libsjb_hip.so")
+
+# enums (include/sjb/windowing/BaseWindowing.cuh:9-40 values)
+ALG = {"EMPTY": 0, "NLJ": 1, "HJ": 2, "SMJ": 3}
+OUT = {"NoOutput": 0, "CountKernel": 1, "EstimatedSelectivity": 2, "Bitmap": 3, "Atomic": 4}
+F_C2, F_GROUPED, F_LEARNED, F_TIMING = 0x1, 0x2, 0x4, 0x8
+DIST_UNIFORM, DIST_ZIPF = 0, 1
+
+E = {
+    0: "SJB_OK", -1: "SJB_E_INVALID", -2: "SJB_E_CAPACITY", -3: "SJB_E_NOMEM", -4: "SJB_E_HIP",
+    -5: "SJB_E_UNSUPPORTED", -6: "SJB_E_NODEVICE", -7: "SJB_E_BUDGET", -8: "SJB_E_TIMEOUT",
+    -9: "SJB_E_STATE",
+}
+
+TUPLE_DTYPE = np.dtype([("key", "<u4"), ("val", "<u4"), ("ts", "<u8")])
+RESULT_DTYPE = np.dtype([("lkey", "<u4"), ("rkey", "<u4"), ("lval", "<u4"), ("rval", "<u4"),
+                         ("lts", "<u8"), ("rts", "<u8")])
+WRES_DTYPE = np.dtype([("count", "<u8"), ("checksum1", "<u8"), ("checksum2", "<u8"),
+                       ("pairs", "<u8")])
+
+
+class SjbError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: {E.get(code, code)}")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "result_count", "build_time_us", "probe_time_us", "sort_time_us", "merge_time_us",
+        "checksum1", "checksum2", "pairs_written", "device_ns")] + [
+        ("c2_valid", ctypes.c_uint32), ("fanout_log2", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("max_window_tuples", ctypes.c_uint64), ("num_distinct_keys", ctypes.c_uint64),
+                ("batch_size", ctypes.c_uint64), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("fanout_log2", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+# exported symbols and their signatures: (restype, argtypes)
+_u64, _u32, _i, _p, _d = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+SIGNATURES = {
+    "sjb_abi_version": (_i, []),
+    "sjb_strerror": (ctypes.c_char_p, [_i]),
+    "sjb_open": (_i, [_i, ctypes.POINTER(Options), ctypes.POINTER(_p)]),
+    "sjb_close": (_i, [_p]),
+    "sjb_stream": (_p, [_p]),
+    "sjb_sync": (_i, [_p]),
+    "sjb_device_bytes": (_u64, [_p]),
+    "sjb_lazy_join": (_i, [_p, _i, _i, _u32, _p, _u64, _p, _u64, _p, _u64, ctypes.POINTER(Stats)]),
+    "sjb_lazy_execute": (_i, [_p, _i, _i, _u32, _p, _p, _u64, _p, _p, _u64, _u64, _p, _u64,
+                              ctypes.POINTER(Stats)]),
+    "sjb_lazy_join_async": (_i, [_p, _i, _u32, _p, _u64, _p, _u64, _p]),
+    "sjb_lazy_join_batch": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _p]),
+    "sjb_eager_execute": (_i, [_p, _i, _i, _u32, _p, _u64, _i, _p, _u64, ctypes.POINTER(Stats)]),
+    "sjb_eager_clear": (_i, [_p]),
+    "sjb_eager_end_of_stream": (_i, [_p]),
+    "sjb_generate": (_i, [_p, _u64, _i, _u64, _d, _u64, _u64]),
+    "sjb_mix64": (_u64, [_u64]),
+    "sjb_hash_a": (_u64, [_u32, _u32]),
+    "sjb_hash_b": (_u64, [_u32, _u32]),
+    "sjb_pair_hash": (_u64, [_u32, _u32, _u32]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libsjb_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsjb_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise SjbError(rc, what)
+    return rc
+
+
+def ptr(a):
+    """Raw address of a numpy array or a torch tensor."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def generate(n, dist=DIST_UNIFORM, key_range=1 << 16, theta=0.0, seed=123, ts_base=0):
+    out = np.empty(n, dtype=TUPLE_DTYPE)
+    check(lib().sjb_generate(ptr(out), n, dist, key_range, theta, seed, ts_base), "sjb_generate")
+    return out
+
+
+class Context:
+    """One sjb_ctx: a HIP stream plus the device buffers of one window size."""
+
+    def __init__(self, device=0, max_window=1 << 22, distinct_keys=1000, batch_size=8192,
+                 hbm_budget=0, fanout_log2=0, flags=0):
+        o = Options(max_window, distinct_keys, batch_size, hbm_budget, fanout_log2, flags)
+        h = ctypes.c_void_p()
+        check(lib().sjb_open(device, ctypes.byref(o), ctypes.byref(h)), "sjb_open")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().sjb_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().sjb_stream(self.h)
+
+    def sync(self):
+        check(lib().sjb_sync(self.h), "sjb_sync")
+
+    def device_bytes(self):
+        return lib().sjb_device_bytes(self.h)
+
+    def lazy_join(self, alg, method, flags, left, n_left, right, n_right, out=None, out_cap=0,
+                  raise_on_error=True):
+        st = Stats()
+        rc = lib().sjb_lazy_join(self.h, alg, method, flags, ptr(left), n_left, ptr(right), n_right,
+                                 ptr(out), out_cap, ctypes.byref(st))
+        if raise_on_error:
+            check(rc, "sjb_lazy_join")
+        return (rc, st) if not raise_on_error else st
+
+    def lazy_execute(self, alg, method, flags, left_ring, left_idx, right_ring, right_idx,
+                     batch_size, out=None, out_cap=0):
+        st = Stats()
+        li = np.ascontiguousarray(left_idx, dtype=np.uint64)
+        ri = np.ascontiguousarray(right_idx, dtype=np.uint64)
+        rc = lib().sjb_lazy_execute(self.h, alg, method, flags, ptr(left_ring), ptr(li), len(li),
+                                    ptr(right_ring), ptr(ri), len(ri), batch_size, ptr(out),
+                                    out_cap, ctypes.byref(st))
+        return rc, st
+
+    def lazy_join_async(self, alg, flags, left, n_left, right, n_right, d_result):
+        check(lib().sjb_lazy_join_async(self.h, alg, flags, ptr(left), n_left, ptr(right), n_right,
+                                        d_result), "sjb_lazy_join_async")
+
+    def lazy_join_batch(self, alg, flags, left, left_off, right, right_off):
+        lo = np.ascontiguousarray(left_off, dtype=np.uint64)
+        ro = np.ascontiguousarray(right_off, dtype=np.uint64)
+        nw = len(lo) - 1
+        res = np.zeros(nw, dtype=WRES_DTYPE)
+        check(lib().sjb_lazy_join_batch(self.h, alg, flags, ptr(left), ptr(lo), ptr(right), ptr(ro),
+                                        nw, ptr(res)), "sjb_lazy_join_batch")
+        return res
+
+    def eager_execute(self, alg, method, flags, batch, is_left, out=None, out_cap=0):
+        st = Stats()
+        rc = lib().sjb_eager_execute(self.h, alg, method, flags, ptr(batch), len(batch),
+                                     1 if is_left else 0, ptr(out), out_cap, ctypes.byref(st))
+        return rc, st
+
+    def eager_clear(self):
+        check(lib().sjb_eager_clear(self.h), "sjb_eager_clear")

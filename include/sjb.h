@@ -1,0 +1,221 @@
+/*
+ * sjb.h — C ABI of the MI355X (gfx950) windowed stream-join operators.
+ *
+ * This is the drop-in boundary for the reference's join-operator layer
+ * (TU-Berlin-DIMA/gpu-stream-join-benchmark).  Every entry point takes plain
+ * pointers and sizes, returns an int status (0 = ok, negative = SJB_E_*),
+ * never exits the process and never throws across the ABI.  The C++ adapters
+ * in gpu-stream-join-benchmark_amd/host/ wrap these calls in the reference's
+ * class interfaces (BaseLazyExecutor / BaseEagerExecutor / Sink / factory).
+ *
+ * Sides: "left" = R = build side, "right" = S = probe side, as in the
+ * reference (src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:122-128).
+ *
+ * Checksums (the reference defines none; see sjb_mix64 below):
+ *   A(k,v)  = mix64(((u64)k << 32 | v) ^ SJB_SEED_A)
+ *   B(k,v)  = mix64(((u64)k << 32 | v) ^ SJB_SEED_B)
+ *   C1      = sum over matched pairs (r,s) of A(k, r.val) * B(k, s.val)  mod 2^64
+ *   H2      = mix64(A(k, r.val) + SJB_GOLDEN * ((u64)s.val << 1 | 1))
+ *   C2      = sum over matched pairs of H2                              mod 2^64
+ * C1 factorises per key group, so it is computable in O(n) at any skew;
+ * C2 needs every pair and is only produced when SJB_F_C2 is requested.
+ */
+#ifndef SJB_H
+#define SJB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SJB_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define SJB_OK 0
+#define SJB_E_INVALID (-1)     /* bad argument / unknown enum value            */
+#define SJB_E_CAPACITY (-2)    /* output or window larger than the buffer/ctx  */
+#define SJB_E_NOMEM (-3)       /* device or host allocation failed             */
+#define SJB_E_HIP (-4)         /* a HIP runtime call failed                    */
+#define SJB_E_UNSUPPORTED (-5) /* combination not offered (mirrors the factory) */
+#define SJB_E_NODEVICE (-6)    /* no usable gfx950 device                      */
+#define SJB_E_BUDGET (-7)      /* HBM budget would be exceeded                 */
+#define SJB_E_TIMEOUT (-8)     /* a bounded wait (persistent kernel) expired   */
+#define SJB_E_STATE (-9)       /* call not valid in the current ctx state      */
+
+/* ---- enums: same values as include/sjb/windowing/BaseWindowing.cuh:9-40 - */
+/* ALGORITHM_TYPE */
+#define SJB_ALG_EMPTY 0
+#define SJB_ALG_NLJ 1
+#define SJB_ALG_HJ 2
+#define SJB_ALG_SMJ 3
+/* RESULT_WRITING_METHOD */
+#define SJB_OUT_NOOUTPUT 0
+#define SJB_OUT_COUNTKERNEL 1
+#define SJB_OUT_ESTIMATEDSELECTIVITY 2
+#define SJB_OUT_BITMAP 3
+#define SJB_OUT_ATOMIC 4
+/* PROGRESSIVENESS_TYPE */
+#define SJB_EAGER 0
+#define SJB_LAZY 1
+/* DEVICE_TYPE */
+#define SJB_DEV_CPU 0
+#define SJB_DEV_GPU 1
+
+/* ---- per-call flags ---------------------------------------------------- */
+#define SJB_F_C2 0x1u          /* also compute C2 (enumerates every pair)      */
+#define SJB_F_GROUPED 0x2u     /* key-group aggregation (count + C1 only);     */
+                               /* O(n) at any skew, C2 not produced            */
+#define SJB_F_LEARNED 0x4u     /* partition with the ctx's learned CDF model   */
+#define SJB_F_TIMING 0x8u      /* record per-phase device time (hipEvents)     */
+
+/* ---- data layout ------------------------------------------------------- */
+/* Identical to Tuple (include/sjb/utils/Tuple.hpp:6-11): 16 B, AoS.        */
+typedef struct sjb_tuple {
+    uint32_t key;
+    uint32_t val;
+    uint64_t ts;
+} sjb_tuple_t;
+
+/* Identical to ResultTuple (include/sjb/utils/ResultTuple.hpp:5-13): 32 B. */
+typedef struct sjb_result_tuple {
+    uint32_t lkey;
+    uint32_t rkey;
+    uint32_t lval;
+    uint32_t rval;
+    uint64_t lts;
+    uint64_t rts;
+} sjb_result_tuple_t;
+
+/* Per-window result: ExecutionStatistic (include/sjb/utils/ExecutionStatistic.hpp:6-20)
+ * plus the checksums and device timings this build adds.                    */
+typedef struct sjb_stats {
+    uint64_t result_count;  /* ExecutionStatistic::resultCount                */
+    uint64_t build_time_us; /* ExecutionStatistic::buildTime  (host clock)    */
+    uint64_t probe_time_us; /* ExecutionStatistic::probeTime                  */
+    uint64_t sort_time_us;  /* ExecutionStatistic::sortTime                   */
+    uint64_t merge_time_us; /* ExecutionStatistic::mergeTime                  */
+    uint64_t checksum1;     /* C1                                              */
+    uint64_t checksum2;     /* C2 (valid when c2_valid)                        */
+    uint64_t pairs_written; /* result tuples stored into the caller's buffer  */
+    uint64_t device_ns;     /* device time of the call (SJB_F_TIMING)         */
+    uint32_t c2_valid;
+    uint32_t fanout_log2;   /* partition fanout actually used                 */
+} sjb_stats_t;
+
+/* Compact per-window result used by the batched and async paths.           */
+typedef struct sjb_window_result {
+    uint64_t count;
+    uint64_t checksum1;
+    uint64_t checksum2;
+    uint64_t pairs; /* pairs written (materialising methods) */
+} sjb_window_result_t;
+
+/* Context options (the reference's executor constructor arguments:
+ * numDistinctKeys, batchSize, maxTupleInWindow — e.g.
+ * src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:266-287).                  */
+typedef struct sjb_options {
+    uint64_t max_window_tuples; /* per-side capacity of one lazy window        */
+    uint64_t num_distinct_keys; /* D: bucket count of the reference CPU/eager  */
+                                /* joins and the ES size estimate (1/D)       */
+    uint64_t batch_size;        /* B: tuples per batch / ring slot             */
+    uint64_t hbm_budget_bytes;  /* 0 = no budget                               */
+    uint32_t fanout_log2;       /* 0 = automatic                               */
+    uint32_t flags;             /* default SJB_F_* applied to every call       */
+} sjb_options_t;
+
+typedef struct sjb_ctx sjb_ctx_t;
+
+/* ---- library / context ------------------------------------------------- */
+int sjb_abi_version(void);
+const char *sjb_strerror(int code);
+
+/* Opens a context on `device`: one HIP stream plus every device buffer a
+ * window of max_window_tuples per side needs (no per-window hipMalloc).
+ * Replaces the executor constructors, e.g. GPU_HJ::GPU_HJ
+ * (src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:266-287).                  */
+int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out);
+int sjb_close(sjb_ctx_t *ctx);
+/* The ctx's HIP stream (hipStream_t) so callers can order or time work.   */
+void *sjb_stream(sjb_ctx_t *ctx);
+int sjb_sync(sjb_ctx_t *ctx);
+/* Device bytes currently held by the ctx.                                  */
+uint64_t sjb_device_bytes(sjb_ctx_t *ctx);
+
+/* ---- lazy (tumbling) window join --------------------------------------- */
+/* One window, inputs already resident in device memory.  Replaces
+ * BaseLazyExecutor::execute (include/sjb/windowing/lazy/executor/BaseLazyExecutor.hpp:26-30)
+ * for the GPU executors GPU_{HJ,SMJ,NLJ}_{NoOutput,CountKernel,
+ * EstimatedSelectivity,Atomic,Bitmap} (src/sjb/windowing/lazy/executor/GPU/).
+ * `out` (host or device pointer, capacity `out_cap` result tuples) is used
+ * by the materialising methods; if the join has more matches than out_cap,
+ * CountKernel/EstimatedSelectivity return SJB_E_CAPACITY with
+ * stats->result_count = required size, and Atomic wraps (pos % out_cap) as
+ * the reference sink ring does (src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:92).
+ * Synchronous: returns when stats are host-visible.                         */
+int sjb_lazy_join(sjb_ctx_t *ctx, int algorithm, int method, uint32_t flags,
+                  const sjb_tuple_t *d_left, uint64_t n_left,
+                  const sjb_tuple_t *d_right, uint64_t n_right,
+                  sjb_result_tuple_t *out, uint64_t out_cap, sjb_stats_t *stats);
+
+/* Same, but gathers the window from host ring-buffer slots: `left_idx` holds
+ * n_left_idx tuple offsets of batch_size-tuple slots in left_ring — exactly
+ * the arguments of BaseLazyExecutor::execute.  The H2D copy is part of the
+ * call, as in the reference (src/sjb/windowing/lazy/executor/GPU/GPU_HJ_NoOutput.cu:37-41). */
+int sjb_lazy_execute(sjb_ctx_t *ctx, int algorithm, int method, uint32_t flags,
+                     const sjb_tuple_t *left_ring, const uint64_t *left_idx, uint64_t n_left_idx,
+                     const sjb_tuple_t *right_ring, const uint64_t *right_idx, uint64_t n_right_idx,
+                     uint64_t batch_size, sjb_result_tuple_t *out, uint64_t out_cap,
+                     sjb_stats_t *stats);
+
+/* Asynchronous count/checksum window (NoOutput): enqueues on the ctx stream
+ * and writes the result into device memory `d_result` (one
+ * sjb_window_result_t).  Used for back-to-back windows with no host sync. */
+int sjb_lazy_join_async(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
+                        const sjb_tuple_t *d_left, uint64_t n_left,
+                        const sjb_tuple_t *d_right, uint64_t n_right,
+                        sjb_window_result_t *d_result);
+
+/* Many uneven windows in one launch sequence (count/checksum only).  Window
+ * w is d_left[left_off[w], left_off[w+1]) x d_right[right_off[w], right_off[w+1]);
+ * offsets are host arrays of n_windows+1 entries.  Results (host array) are
+ * identical to n_windows calls of sjb_lazy_join.                          */
+int sjb_lazy_join_batch(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
+                        const sjb_tuple_t *d_left, const uint64_t *left_off,
+                        const sjb_tuple_t *d_right, const uint64_t *right_off,
+                        uint32_t n_windows, sjb_window_result_t *results);
+
+/* ---- eager (per-batch) join -------------------------------------------- */
+/* Joins one incoming batch against the opposite side's resident window state
+ * and then appends it to its own side's state.  Replaces
+ * BaseEagerExecutor::execute/clearStates/onEndOfStream
+ * (include/sjb/windowing/eager/executor/BaseEagerExecutor.hpp:16-20) for the
+ * executors GPU_HJ, GPU_HJ_Atomic, GPU_NLJ, GPU_NLJ_BM
+ * (src/sjb/windowing/eager/executor/GPU/).  `batch` is a host pointer.     */
+int sjb_eager_execute(sjb_ctx_t *ctx, int algorithm, int method, uint32_t flags,
+                      const sjb_tuple_t *batch, uint64_t n, int is_left,
+                      sjb_result_tuple_t *out, uint64_t out_cap, sjb_stats_t *stats);
+int sjb_eager_clear(sjb_ctx_t *ctx);
+int sjb_eager_end_of_stream(sjb_ctx_t *ctx);
+
+/* ---- data generation (host, deterministic) ----------------------------- */
+/* Fills n tuples: key distribution by `dist` (0 uniform over [0,key_range),
+ * 1 Zipf(theta) over [0,key_range) with ranks scrambled by a fixed
+ * bijection), val = independent uniform u32, ts = ts_base + index.        */
+#define SJB_DIST_UNIFORM 0
+#define SJB_DIST_ZIPF 1
+int sjb_generate(sjb_tuple_t *out, uint64_t n, int dist, uint64_t key_range, double theta,
+                 uint64_t seed, uint64_t ts_base);
+
+/* ---- checksum primitives (host reference of the device formulas) ------ */
+uint64_t sjb_mix64(uint64_t z);
+uint64_t sjb_hash_a(uint32_t key, uint32_t val);
+uint64_t sjb_hash_b(uint32_t key, uint32_t val);
+uint64_t sjb_pair_hash(uint32_t key, uint32_t rval, uint32_t sval);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SJB_H */

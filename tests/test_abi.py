@@ -1,0 +1,112 @@
+"""The C-ABI library: loads, exports every declared symbol, host-side logic.
+
+No compute call reaches a GPU here; sjb_open must fail with SJB_E_NODEVICE
+rather than crash when no device is present.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, _gpu_available
+
+
+def header_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if not name.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", name)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(sjb_[a-z0-9_]+)\s*\(", txt):
+            syms.add(m.group(1))
+    return syms
+
+
+def test_exports_every_declared_symbol(sjbmod):
+    L = ctypes.CDLL(sjbmod.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in sorted(syms) if not hasattr(L, s)]
+    assert not missing, missing
+    # every exported entry point has a binding signature
+    assert syms <= set(sjbmod.SIGNATURES) | {"sjb_ctx"}
+
+
+def test_abi_version_and_strerror(sjbmod):
+    L = sjbmod.lib()
+    assert L.sjb_abi_version() == 1
+    assert L.sjb_strerror(-2) == b"capacity exceeded"
+    assert L.sjb_strerror(12345) == b"unknown error"
+
+
+def test_struct_layouts(sjbmod):
+    assert sjbmod.TUPLE_DTYPE.itemsize == 16
+    assert sjbmod.RESULT_DTYPE.itemsize == 32
+    assert ctypes.sizeof(sjbmod.Stats) == 9 * 8 + 8
+    assert ctypes.sizeof(sjbmod.Options) == 4 * 8 + 8
+
+
+@pytest.mark.skipif(_gpu_available(), reason="checks the no-device error path")
+def test_open_without_device_fails_cleanly(sjbmod):
+    with pytest.raises(sjbmod.SjbError) as ei:
+        sjbmod.Context(device=0)
+    assert ei.value.code == -6
+
+
+def test_null_arguments_rejected(sjbmod):
+    L = sjbmod.lib()
+    st = sjbmod.Stats()
+    assert L.sjb_lazy_join(None, 2, 0, 0, None, 0, None, 0, None, 0, ctypes.byref(st)) == -1
+    assert L.sjb_close(None) == -1
+    assert L.sjb_eager_clear(None) == -1
+    assert L.sjb_generate(None, 10, 0, 100, 0.0, 1, 0) == -1
+
+
+def test_checksum_primitives_match_oracle(sjbmod, orc):
+    L, O = sjbmod.lib(), orc.lib()
+    rng = np.random.default_rng(3)
+    for k, v, w in rng.integers(0, 2**32, (200, 3), dtype=np.uint64):
+        k, v, w = int(k), int(v), int(w)
+        assert L.sjb_hash_a(k, v) == O.or_hash_a(k, v)
+        assert L.sjb_hash_b(k, v) == O.or_hash_b(k, v)
+        assert L.sjb_pair_hash(k, v, w) == O.or_pair_hash(k, v, w)
+
+
+def test_generator_uniform(sjbmod):
+    a = sjbmod.generate(1 << 17, sjbmod.DIST_UNIFORM, 1 << 10, 0.0, seed=123, ts_base=5)
+    b = sjbmod.generate(1 << 17, sjbmod.DIST_UNIFORM, 1 << 10, 0.0, seed=123, ts_base=5)
+    c = sjbmod.generate(1 << 17, sjbmod.DIST_UNIFORM, 1 << 10, 0.0, seed=456, ts_base=5)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a["key"], c["key"])
+    assert a["key"].max() < 1 << 10
+    assert np.array_equal(a["ts"], np.arange(5, 5 + (1 << 17), dtype=np.uint64))
+    h = np.bincount(a["key"], minlength=1 << 10)
+    exp = (1 << 17) / (1 << 10)
+    assert abs(h - exp).max() < 6 * np.sqrt(exp)
+    # val independent of key (the reference sets val = key)
+    assert np.mean(a["val"] == a["key"]) < 0.01
+    # thread split does not change bytes: a prefix equals a shorter draw
+    d = sjbmod.generate(1000, sjbmod.DIST_UNIFORM, 1 << 10, 0.0, seed=123, ts_base=5)
+    assert np.array_equal(a[:1000], d)
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0, 1.2])
+def test_generator_zipf_frequencies(sjbmod, theta):
+    K, n = 64, 1 << 18
+    a = sjbmod.generate(n, sjbmod.DIST_ZIPF, K, theta, seed=9)
+    assert a["key"].max() < K
+    freq = np.sort(np.bincount(a["key"], minlength=K))[::-1] / n
+    p = np.arange(1, K + 1, dtype=np.float64) ** -theta
+    p /= p.sum()
+    # frequency of the k-th most frequent key ~ p_k (ranks scrambled to keys)
+    assert np.allclose(freq[:8], p[:8], rtol=0.05, atol=2e-3)
+
+
+def test_generator_scramble_is_bijective(sjbmod):
+    # theta -> 0 Zipf is uniform over ranks; with a tiny key range every key
+    # must appear (a non-bijective scramble would leave holes)
+    a = sjbmod.generate(1 << 14, sjbmod.DIST_ZIPF, 100, 1e-9, seed=4)
+    assert len(np.unique(a["key"])) == 100

@@ -1,0 +1,195 @@
+"""GPU parity: the HIP operators (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact count, C1 and C2 (integer arithmetic mod 2^64) and identical
+sorted pair lists wherever pairs are produced.
+"""
+import numpy as np
+import pytest
+
+from conftest import to_dev
+from helpers import make, np_join, np_pair_rows, sort_rows, zipf_keys
+
+pytestmark = pytest.mark.gpu
+
+HJ, SMJ, NLJ = 2, 3, 1
+NO, CK, ES, BM, AT = 0, 1, 2, 3, 4
+F_C2, F_GROUPED = 1, 2
+
+SIZES = [(1, 1), (100, 37), (5000, 4096), (16389, 20011), (70000, 65536)]
+
+
+def run_count(ctx, torch, alg, R, S, flags=F_C2):
+    dR, dS = to_dev(torch, R), to_dev(torch, S)
+    st = ctx.lazy_join(alg, NO, flags, dR, len(R), dS, len(S))
+    return st
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
+@pytest.mark.parametrize("key_range", [16, 1000, 1 << 16, 1 << 24])
+@pytest.mark.parametrize("size", SIZES)
+def test_count_checksums(ctx, torch_mod, orc, alg, key_range, size):
+    nL, nR = size
+    if alg == NLJ and nL * nR > 1 << 32:
+        pytest.skip("NLJ size")
+    print(f"case alg={alg} range={key_range} size={size}", flush=True)
+    R, S = make(nL, key_range, 1000 + nL), make(nR, key_range, 2000 + nR)
+    ref = np_join(R, S) if nL * nR < 1 << 26 or key_range > 1 << 12 else None
+    o, _ = orc.lazy_hj(R, S, 1 << 16)
+    if ref is not None:
+        assert o.triple() == ref
+    st = run_count(ctx, torch_mod, alg, R, S)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    assert st.c2_valid == 1
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+@pytest.mark.parametrize("theta", [0.8, 1.2])
+def test_skewed_keys(ctx, torch_mod, orc, alg, theta):
+    n = 1 << 16
+    print(f"case skew alg={alg} theta={theta}", flush=True)
+    R = make(n, 1, 11, keys=zipf_keys(n, theta, 1 << 14, 1))
+    S = make(n, 1, 12, keys=zipf_keys(n, theta, 1 << 14, 2))
+    o, _ = orc.lazy_smj(R, S)
+    flags = F_GROUPED if alg == SMJ else 0
+    st = run_count(ctx, torch_mod, alg, R, S, flags)
+    assert (st.result_count, st.checksum1) == (o.count, o.c1)
+
+
+def test_grouped_equals_pairwise(ctx, torch_mod):
+    R, S = make(1 << 18, 1 << 12, 5), make(1 << 18, 1 << 12, 6)
+    a = run_count(ctx, torch_mod, SMJ, R, S, F_GROUPED)
+    b = run_count(ctx, torch_mod, SMJ, R, S, F_C2)
+    c = run_count(ctx, torch_mod, HJ, R, S, F_C2)
+    assert (a.result_count, a.checksum1) == (b.result_count, b.checksum1)
+    assert (b.result_count, b.checksum1, b.checksum2) == (c.result_count, c.checksum1, c.checksum2)
+
+
+@pytest.mark.parametrize("alg,method", [(HJ, CK), (HJ, ES), (HJ, AT), (SMJ, CK), (SMJ, ES),
+                                        (SMJ, AT), (NLJ, CK), (NLJ, ES), (NLJ, AT), (NLJ, BM)])
+def test_materialised_pairs(ctx, torch_mod, orc, alg, method):
+    print(f"case pairs alg={alg} method={method}", flush=True)
+    R, S = make(3000, 500, 71), make(2500, 500, 72)
+    ref_rows = sort_rows(np_pair_rows(R, S))
+    o, _ = orc.lazy_nlj(R, S)
+    cap = len(ref_rows) + 10
+    out = torch_mod.zeros(cap * 32, dtype=torch_mod.uint8, device="cuda")
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    st = ctx.lazy_join(alg, method, 0, dR, len(R), dS, len(S), out, cap)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    assert st.pairs_written == len(ref_rows)
+    rows = out.cpu().numpy().view(ref_rows.dtype)[: st.pairs_written]
+    assert np.array_equal(sort_rows(rows), ref_rows)
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
+def test_capacity_errors(ctx, torch_mod, alg):
+    R, S = make(2000, 50, 81), make(2000, 50, 82)
+    n_ref = np_join(R, S)[0]
+    out = torch_mod.zeros(100 * 32, dtype=torch_mod.uint8, device="cuda")
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    for method in (CK, ES):
+        rc, st = ctx.lazy_join(alg, method, 0, dR, len(R), dS, len(S), out, 100,
+                               raise_on_error=False)
+        assert rc == -2 and st.result_count == n_ref
+    # Atomic wraps like the reference's sink ring: no error, count exact
+    rc, st = ctx.lazy_join(alg, AT, 0, dR, len(R), dS, len(S), out, 100, raise_on_error=False)
+    assert rc == 0 and st.result_count == n_ref and st.pairs_written == 100
+    # window larger than the context
+    rc, st = ctx.lazy_join(alg, NO, 0, dR, (1 << 22) + 1, dS, 10, raise_on_error=False)
+    assert rc == -2
+    rc, st = ctx.lazy_join(SMJ if alg == HJ else HJ, BM, 0, dR, 10, dS, 10, out, 100,
+                           raise_on_error=False)
+    assert rc == -5
+
+
+@pytest.mark.parametrize("logF", [1, 4, 9, 13])
+def test_fanout_invariance(sjbmod, torch_mod, orc, logF):
+    R, S = make(50000, 1 << 14, 91), make(60000, 1 << 14, 92)
+    o, _ = orc.lazy_hj(R, S, 1 << 14)
+    with sjbmod.Context(max_window=1 << 17, fanout_log2=logF) as c:
+        for alg in (HJ, SMJ):
+            st = run_count(c, torch_mod, alg, R, S)
+            assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+            assert st.fanout_log2 == logF
+
+
+def test_lazy_execute_ring_slots(ctx, orc):
+    B = 512
+    ringL, ringR = make(64 * B, 300, 101), make(64 * B, 300, 102)
+    li = np.array([5, 6, 7, 40, 2, 3], dtype=np.uint64) * B  # non-adjacent slots
+    ri = np.array([63, 0, 1, 10, 11, 12], dtype=np.uint64) * B
+    L = np.concatenate([ringL[i:i + B] for i in li])
+    R = np.concatenate([ringR[i:i + B] for i in ri])
+    o, _ = orc.lazy_hj(L, R, 1000)
+    for alg in (HJ, SMJ, NLJ):
+        rc, st = ctx.lazy_execute(alg, NO, F_C2, ringL, li, ringR, ri, B)
+        assert rc == 0
+        assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+
+
+def test_batch_equals_single(ctx, torch_mod, orc):
+    rng = np.random.default_rng(5)
+    sizes_l = rng.integers(0, 40000, 12)
+    sizes_r = rng.integers(0, 40000, 12)
+    sizes_l[3] = 0
+    R = make(int(sizes_l.sum()), 5000, 111)
+    S = make(int(sizes_r.sum()), 5000, 112)
+    lo = np.concatenate([[0], np.cumsum(sizes_l)]).astype(np.uint64)
+    so = np.concatenate([[0], np.cumsum(sizes_r)]).astype(np.uint64)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    for alg in (HJ, SMJ):
+        res = ctx.lazy_join_batch(alg, F_C2, dR, lo, dS, so)
+        for w in range(12):
+            o, _ = orc.lazy_hj(R[lo[w]:lo[w + 1]], S[so[w]:so[w + 1]], 1000)
+            assert (int(res[w]["count"]), int(res[w]["checksum1"]), int(res[w]["checksum2"])) == o.triple()
+
+
+@pytest.mark.parametrize("alg", [HJ, NLJ])
+def test_eager_vs_oracle(sjbmod, orc, alg):
+    B, nb = 1000, 10
+    W = B * nb
+    R, S = make(W, 400, 121), make(W, 400, 122)
+    e = orc.Eager(1000, W)
+    with sjbmod.Context(max_window=W) as c:
+        for rnd in range(2):  # two windows: clear between them
+            for b in range(nb):
+                for is_left, src in ((1, R), (0, S)):
+                    batch = src[b * B:(b + 1) * B]
+                    rc, st = c.eager_execute(alg, NO, F_C2, batch, is_left)
+                    assert rc == 0
+                    o = e.execute(batch, is_left)
+                    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+            c.eager_clear()
+            e.clear()
+        # capacity: a full side rejects one more batch
+        for b in range(nb):
+            c.eager_execute(alg, NO, 0, R[b * B:(b + 1) * B], 1)
+        rc, _ = c.eager_execute(alg, NO, 0, R[:B], 1)
+        assert rc == -2
+
+
+def test_eager_atomic_pairs(sjbmod, orc, torch_mod):
+    B = 700
+    R, S = make(4 * B, 200, 131), make(4 * B, 200, 132)
+    with sjbmod.Context(max_window=4 * B) as c:
+        rows = []
+        for b in range(4):
+            for is_left, src in ((1, R), (0, S)):
+                out = np.zeros(B * 4 * B, dtype=sjbmod.RESULT_DTYPE)
+                rc, st = c.eager_execute(HJ, AT, 0, src[b * B:(b + 1) * B], is_left, out, len(out))
+                assert rc == 0
+                rows.append(out[: st.pairs_written])
+        got = sort_rows(np.concatenate(rows))
+        assert np.array_equal(got, sort_rows(np_pair_rows(R, S)))
+
+
+def test_full_size_window(ctx, torch_mod, orc, sjbmod):
+    # 2^22 tuples per side, keys uniform in [0, 2^24): HJ == SMJ == oracle.
+    n = 1 << 22
+    R = sjbmod.generate(n, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=123)
+    S = sjbmod.generate(n, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=456)
+    o, _ = orc.lazy_hj(R, S, 1 << 20)
+    for alg in (HJ, SMJ):
+        st = run_count(ctx, torch_mod, alg, R, S)
+        assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    assert 0.9 * 2**20 < o.count < 1.1 * 2**20
