@@ -127,6 +127,7 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     p.payload = write ? PAY_IDX : PAY_VAL;
     p.ncL = int((nL + kChunk - 1) / kChunk);
     p.ncR = int((nR + kChunk - 1) / kChunk);
+    if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
     *logF_used = p.logF;
     if (p.kind == PART_RANGE) {
         rc = launch_minmax(c, L, nL, R, nR, p.logF);
@@ -312,16 +313,15 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
         const uint64_t need = cap * (2 * sizeof(sjb_tuple_t) + 2 * sizeof(Rec));
         if (need > c->opts.hbm_budget_bytes) return fail(SJB_E_BUDGET);
     }
-    const uint64_t nchunks = (cap + kChunk - 1) / kChunk;
-    c->hist_cap = (uint64_t(1) << kMaxLogF) * 2 * std::max<uint64_t>(nchunks, 1);
-    const uint64_t scan_blocks = c->hist_cap / 4096 + 2048;
+    const uint64_t nchunks = std::min<uint64_t>((cap + kChunk - 1) / kChunk, kMaxChunks);
+    c->offs_cap = ((uint64_t(1) << kMaxLogF) + 1) * 2 * std::max<uint64_t>(nchunks, 1);
     const uint64_t n_partials = std::max<uint64_t>(65536, 1ull << kMaxLogF);
     c->mask_cap = std::max<uint64_t>(cap, 1ull << 20);
     if ((rc = dev_alloc(c, (void **)&c->d_stage[0], cap * sizeof(sjb_tuple_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_stage[1], cap * sizeof(sjb_tuple_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_part, 2 * cap * sizeof(Rec))) ||
-        (rc = dev_alloc(c, (void **)&c->d_hist, c->hist_cap * sizeof(uint32_t))) ||
-        (rc = dev_alloc(c, (void **)&c->d_scan_blocks, scan_blocks * sizeof(uint32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_offs, c->offs_cap * sizeof(uint32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->d_scratch, 4096 * sizeof(uint32_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_partials, 3 * n_partials * sizeof(uint64_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_result, 64 * sizeof(sjb_window_result_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_pparams, sizeof(PartParams))) ||
@@ -331,6 +331,7 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
         (rc = host_alloc(c, (void **)&c->h_result, 64 * sizeof(sjb_window_result_t))))
         return fail(rc);
     if (launch_fill_u64(c->stream, c->d_cursor, 0, 4) != SJB_OK) return fail(SJB_E_HIP);
+    partition_setup(c);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SJB_E_HIP);
     *out = c;
     return SJB_OK;

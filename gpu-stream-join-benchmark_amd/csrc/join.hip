@@ -1,8 +1,15 @@
 // join.hip — per-partition join kernels of the lazy HJ and SMJ operators.
 //
-// After partition.hip, partition p of the window is left records
-// d_part[lb, le) and right records d_part[rb, re).  One workgroup joins one
-// partition entirely in LDS:
+// After k_partition (partition.hip), partition p of a window is the union
+// over chunks c of the runs [offs[c][p], offs[c][p+1]) of chunk c's
+// partition-sorted records.  One workgroup joins one partition in LDS:
+//
+//   * run table: the workgroup reads the run bounds of every chunk of both
+//     sides (L2-resident: the offsets array is a few MiB) and scans the run
+//     lengths, so record j of the partition is found by a binary search
+//     over the run prefix;
+//   * records are gathered in batches of kBatch loads per thread, issued
+//     before any of them is used, so one memory latency covers a batch;
 //
 //   HJ  (replaces buildHashTable + probeCount/probeWrite/probeJoin,
 //        src/sjb/windowing/lazy/executor/GPU/GPU_HJ.cu:22-133): the build
@@ -21,7 +28,7 @@
 // (each probe record meets every tile), so any partition size is correct.
 //
 // Materialising variants (payload = tuple index) reserve output slots per
-// round with one global atomic per workgroup (block scan of the lanes'
+// batch with one global atomic per workgroup (block scan of the lanes'
 // match counts), gather both tuples and write ResultTuple rows; they replace
 // the per-match/per-thread atomics of probeWrite/probeJoin
 // (GPU_HJ.cu:87,123).  `wrap` writes pos % cap (the Atomic method's ring,
@@ -31,25 +38,102 @@
 
 namespace sjb {
 
-constexpr int kHjLogNB = 12;
+constexpr int kHjLogNB = 11;
 constexpr int kHjNB = 1 << kHjLogNB;
 constexpr uint32_t kNil32 = 0xFFFFFFFFu;
 constexpr uint16_t kNil16 = 0xFFFFu;
+constexpr int kBatch = 8;
+constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
 
-struct PartRange {
-    uint64_t lb, le, rb, re;
+// Record index of element j (< pref[nc]) by binary search over the run
+// prefix (used only past the run-id table).
+__device__ __forceinline__ uint32_t run_index(const uint32_t *pref, const uint32_t *start, int nc,
+                                              uint32_t j) {
+    int lo = 0, hi = nc;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pref[mid] <= j)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return start[lo] + (j - pref[lo]);
+}
+
+// One partition of a window as seen from its join workgroup: for each side
+// the start of every chunk's run in d_part, the exclusive prefix of the run
+// lengths, and a run-id table (element j -> run) for the first `idcap`
+// elements, so a record index costs three LDS reads.
+struct PartView {
+    uint32_t *start[2];
+    uint32_t *pref[2];
+    uint16_t *runid[2];
+    int nc[2];
+    uint32_t n[2];
+    uint32_t idcap;
+    __device__ __forceinline__ uint32_t index(int side, uint32_t j) const {
+        if (j < idcap) {
+            const uint32_t c = runid[side][j];
+            return start[side][c] + (j - pref[side][c]);
+        }
+        return run_index(pref[side], start[side], nc[side], j);
+    }
 };
 
-__device__ __forceinline__ PartRange part_range(const uint32_t *__restrict__ hist, int logF,
-                                                int ncL, int ncR, uint64_t nL, uint64_t nR, int p) {
-    const int F = 1 << logF;
-    PartRange r;
-    r.lb = hist[uint64_t(p) * ncL];
-    r.le = p + 1 < F ? hist[uint64_t(p + 1) * ncL] : nL;
-    const uint32_t *hr = hist + uint64_t(F) * ncL;
-    r.rb = hr[uint64_t(p) * ncR];
-    r.re = p + 1 < F ? hr[uint64_t(p + 1) * ncR] : nL + nR;
-    return r;
+// Dynamic LDS bytes of a PartView: start/pref for both sides plus two
+// run-id tables of idcap entries.
+__host__ __device__ constexpr size_t part_view_bytes(int ncL, int ncR, int idcap) {
+    return size_t(4) * (size_t(ncL) + ncR + (ncL + 1) + (ncR + 1)) + size_t(2) * 2 * idcap + 16;
+}
+
+// Builds the PartView of partition p in `dyn` (dynamic LDS); ends with a
+// barrier.
+__device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__restrict__ offs,
+                                              int logF, int p, int ncL, int ncR, uint64_t nL,
+                                              uint32_t idcap, uint32_t *ws) {
+    PartView v;
+    v.nc[0] = ncL;
+    v.nc[1] = ncR;
+    v.start[0] = dyn;
+    v.start[1] = dyn + ncL;
+    v.pref[0] = dyn + ncL + ncR;
+    v.pref[1] = v.pref[0] + ncL + 1;
+    v.runid[0] = reinterpret_cast<uint16_t *>(v.pref[1] + ncR + 1);
+    v.runid[1] = v.runid[0] + idcap;
+    v.idcap = idcap;
+    const int F1 = (1 << logF) + 1;
+    for (int side = 0; side < 2; side++) {
+        const int nc = v.nc[side];
+        const uint32_t *o = offs + uint64_t(side ? ncL : 0) * F1;
+        const uint64_t rec_base = side ? nL : 0;
+        uint32_t carry = 0;
+        for (int c0 = 0; c0 < nc; c0 += blockDim.x) {
+            const int c = c0 + threadIdx.x;
+            uint32_t len = 0;
+            if (c < nc) {
+                const uint32_t a = o[uint64_t(c) * F1 + p];
+                const uint32_t b = o[uint64_t(c) * F1 + p + 1];
+                v.start[side][c] = uint32_t(rec_base + uint64_t(c) * kChunk + a);
+                len = b - a;
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_u32(len, ws, &tot);
+            if (c < nc) v.pref[side][c] = carry + ex;
+            carry += tot;
+        }
+        if (threadIdx.x == 0) v.pref[side][nc] = carry;
+        v.n[side] = carry;
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; side++) {
+        for (int c = threadIdx.x; c < v.nc[side]; c += blockDim.x) {
+            const uint32_t a = v.pref[side][c];
+            const uint32_t b = min(v.pref[side][c + 1], idcap);
+            for (uint32_t j = a; j < b; j++) v.runid[side][j] = uint16_t(c);
+        }
+    }
+    __syncthreads();
+    return v;
 }
 
 __device__ __forceinline__ void write_row(sjb_result_tuple_t *__restrict__ out, uint64_t pos,
@@ -72,83 +156,179 @@ __device__ __forceinline__ void write_row(sjb_result_tuple_t *__restrict__ out, 
 // ---------------------------------------------------------------------------
 // HJ
 // ---------------------------------------------------------------------------
+// Partition handled by workgroup `bid`.  Workgroups are dealt round-robin
+// over the 8 XCDs of the MI355X; giving each XCD a contiguous range of
+// partitions makes the runs of neighbouring partitions (which share cache
+// lines inside every chunk) and their offset words hit the same L2.
+__device__ __forceinline__ int xcd_partition(int bid, int F) {
+    constexpr int kXcd = 8;
+    if (F % kXcd) return bid;
+    return (bid % kXcd) * (F / kXcd) + bid / kXcd;
+}
+
+// LDS chained table of one build tile.
+struct HjTable {
+    uint32_t key[kHjCap];
+    uint32_t pay[kHjCap];
+    uint16_t next[kHjCap];
+    uint32_t head[kHjNB];
+};
+
+__device__ __forceinline__ uint32_t hj_bucket(uint32_t k, int bshift) {
+    return (key_hash(k) << bshift) >> (32 - kHjLogNB);
+}
+
+__device__ __forceinline__ void hj_insert(HjTable &t, uint32_t i, Rec r, int bshift) {
+    t.key[i] = r.x;
+    t.pay[i] = r.y;
+    t.next[i] = uint16_t(atomicExch(&t.head[hj_bucket(r.x, bshift)], i));
+}
+
+// Gathers elements [base, base+n) of one side of the partition, kBatch
+// loads per thread in flight, and hands each record to f(i, rec).
+template <class Fn>
+__device__ __forceinline__ void gather(const Rec *__restrict__ part, const PartView &v, int side,
+                                       uint32_t base, uint32_t n, Fn &&f) {
+    for (uint32_t i0 = 0; i0 < n; i0 += kJoinThreads * kBatch) {
+        Rec r[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const uint32_t i = i0 + k * kJoinThreads + threadIdx.x;
+            if (i < n) r[k] = part[v.index(side, base + i)];
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const uint32_t i = i0 + k * kJoinThreads + threadIdx.x;
+            if (i < n) f(i, r[k]);
+        }
+    }
+}
+
+// count / C1 / C2 of one probe record (payload = val) against the tile
+template <bool C2>
+__device__ __forceinline__ void hj_probe_val(const HjTable &t, Rec s, int bshift, Acc &acc) {
+    for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e]) {
+        if (t.key[e] == s.x) {
+            const uint64_t a = hash_a(s.x, t.pay[e]);
+            acc.count++;
+            acc.c1 += a * hash_b(s.x, s.y);
+            if (C2) acc.c2 += pair_hash_from_a(a, s.y);
+        }
+    }
+}
+
+// Probes the un records staged in LDS against the current tile.  WRITE
+// (payload = tuple index): per round one block scan of the lanes' match
+// counts and one global reservation, then rows are written.
+template <bool C2, bool WRITE>
+__device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, uint32_t un, int bshift,
+                                             Acc &acc, const sjb_tuple_t *__restrict__ L,
+                                             const sjb_tuple_t *__restrict__ R,
+                                             sjb_result_tuple_t *__restrict__ out, uint64_t cap, int wrap,
+                                             unsigned long long *__restrict__ cursor, uint32_t *ws,
+                                             unsigned long long *s_base) {
+    if (!WRITE) {
+#pragma unroll 2
+        for (uint32_t j = threadIdx.x; j < un; j += kJoinThreads) hj_probe_val<C2>(t, sp[j], bshift, acc);
+        return;
+    }
+    for (uint32_t j0 = 0; j0 < un; j0 += kJoinThreads) {
+        const uint32_t j = j0 + threadIdx.x;
+        Rec s = make_uint2(0, 0);
+        uint32_t m = 0;
+        if (j < un) {
+            s = sp[j];
+            for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e])
+                m += t.key[e] == s.x;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_u32(m, ws, &tot);
+        if (threadIdx.x == 0) *s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        if (m) {
+            uint64_t pos = *s_base + ex;
+            const sjb_tuple_t st = R[s.y];
+            for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e]) {
+                if (t.key[e] != s.x) continue;
+                const sjb_tuple_t lt = L[t.pay[e]];
+                const uint64_t a = hash_a(lt.key, lt.val);
+                acc.count++;
+                acc.c1 += a * hash_b(st.key, st.val);
+                if (C2) acc.c2 += pair_hash_from_a(a, st.val);
+                write_row(out, pos++, cap, wrap, lt, st);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per partition.  The first build tile and the first kHjCap
+// probe records are gathered with all loads in flight at once (build
+// records go straight into the LDS table, probe records into an LDS
+// staging buffer); larger partitions continue tile by tile.
 template <bool C2, bool WRITE>
 __global__ __launch_bounds__(kJoinThreads) void k_hj_join(
-    const Rec *__restrict__ part, const uint32_t *__restrict__ hist, int logF, int ncL, int ncR,
-    uint64_t nL, uint64_t nR, int bshift, const sjb_tuple_t *__restrict__ L,
-    const sjb_tuple_t *__restrict__ R, uint64_t *__restrict__ partials,
-    sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, int wrap,
-    unsigned long long *__restrict__ cursor) {
-    __shared__ uint32_t s_key[kHjCap];
-    __shared__ uint32_t s_pay[kHjCap];
-    __shared__ uint16_t s_next[kHjCap];
-    __shared__ uint32_t s_head[kHjNB];
+    const Rec *__restrict__ part, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR,
+    uint64_t nL, int bshift, const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
+    uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out, uint64_t out_cap,
+    int wrap, unsigned long long *__restrict__ cursor) {
+    __shared__ HjTable t;
+    __shared__ Rec s_probe[kHjCap];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ unsigned long long s_base;
+    extern __shared__ uint32_t dyn[];
 
-    const int p = blockIdx.x;
-    const PartRange pr = part_range(hist, logF, ncL, ncR, nL, nR, p);
+    const int p = xcd_partition(blockIdx.x, 1 << logF);
+    for (int i = threadIdx.x; i < kHjNB; i += kJoinThreads) t.head[i] = kNil32;
+    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, nL, kHjCap, s_ws);
+    const uint32_t nb = v.n[0], np = v.n[1];
     Acc acc;
     acc.zero();
-
-    for (uint64_t t0 = pr.lb; t0 < pr.le; t0 += kHjCap) {
-        const int tn = int(min<uint64_t>(kHjCap, pr.le - t0));
-        for (int i = threadIdx.x; i < kHjNB; i += kJoinThreads) s_head[i] = kNil32;
-        __syncthreads();
-        for (int i = threadIdx.x; i < tn; i += kJoinThreads) {
-            const Rec r = part[t0 + i];
-            s_key[i] = r.x;
-            s_pay[i] = r.y;
-            const uint32_t b = (key_hash(r.x) << bshift) >> (32 - kHjLogNB);
-            s_next[i] = uint16_t(atomicExch(&s_head[b], uint32_t(i)));
-        }
-        __syncthreads();
-        if (!WRITE) {
-            for (uint64_t j = pr.rb + threadIdx.x; j < pr.re; j += kJoinThreads) {
-                const Rec s = part[j];
-                const uint32_t b = (key_hash(s.x) << bshift) >> (32 - kHjLogNB);
-                for (uint32_t e = s_head[b]; e < kNil16; e = s_next[e]) {
-                    if (s_key[e] == s.x) {
-                        const uint64_t a = hash_a(s.x, s_pay[e]);
-                        acc.count++;
-                        acc.c1 += a * hash_b(s.x, s.y);
-                        if (C2) acc.c2 += pair_hash_from_a(a, s.y);
-                    }
-                }
+    if (nb && np) {
+        const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjCap);
+        {
+            Rec rb[kHjPerT], rp[kHjPerT];
+#pragma unroll
+            for (int k = 0; k < kHjPerT; k++) {
+                const uint32_t i = k * kJoinThreads + threadIdx.x;
+                if (i < tn0) rb[k] = part[v.index(0, i)];
             }
-        } else {
-            // Rounds of one probe record per thread: count, reserve, write.
-            for (uint64_t j0 = pr.rb; j0 < pr.re; j0 += kJoinThreads) {
-                const uint64_t j = j0 + threadIdx.x;
-                Rec s = make_uint2(0, 0);
-                uint32_t b = 0, m = 0;
-                if (j < pr.re) {
-                    s = part[j];
-                    b = (key_hash(s.x) << bshift) >> (32 - kHjLogNB);
-                    for (uint32_t e = s_head[b]; e < kNil16; e = s_next[e]) m += s_key[e] == s.x;
-                }
-                uint32_t tot;
-                uint32_t ex = block_excl_scan_u32(m, s_ws, &tot);
-                if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
-                __syncthreads();
-                if (m) {
-                    uint64_t pos = s_base + ex;
-                    const sjb_tuple_t st = R[s.y];
-                    for (uint32_t e = s_head[b]; e < kNil16; e = s_next[e]) {
-                        if (s_key[e] != s.x) continue;
-                        const sjb_tuple_t lt = L[s_pay[e]];
-                        const uint64_t a = hash_a(lt.key, lt.val);
-                        acc.count++;
-                        acc.c1 += a * hash_b(st.key, st.val);
-                        if (C2) acc.c2 += pair_hash_from_a(a, st.val);
-                        write_row(out, pos++, out_cap, wrap, lt, st);
-                    }
-                }
-                __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kHjPerT; k++) {
+                const uint32_t j = k * kJoinThreads + threadIdx.x;
+                if (j < pn0) rp[k] = part[v.index(1, j)];
+            }
+#pragma unroll
+            for (int k = 0; k < kHjPerT; k++) {
+                const uint32_t i = k * kJoinThreads + threadIdx.x;
+                if (i < tn0) hj_insert(t, i, rb[k], bshift);
+                if (i < pn0) s_probe[i] = rp[k];
             }
         }
         __syncthreads();
+        uint32_t resident = 0;  // first element of the probe chunk in s_probe
+        for (uint32_t t0 = 0; t0 < nb; t0 += kHjCap) {
+            if (t0 > 0) {
+                for (int i = threadIdx.x; i < kHjNB; i += kJoinThreads) t.head[i] = kNil32;
+                __syncthreads();
+                gather(part, v, 0, t0, min<uint32_t>(kHjCap, nb - t0),
+                       [&](uint32_t i, Rec r) { hj_insert(t, i, r, bshift); });
+                __syncthreads();
+            }
+            for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
+                const uint32_t un = min<uint32_t>(kHjCap, np - u0);
+                if (u0 != resident) {
+                    __syncthreads();
+                    gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { s_probe[i] = r; });
+                    resident = u0;
+                    __syncthreads();
+                }
+                hj_probe_lds<C2, WRITE>(t, s_probe, un, bshift, acc, L, R, out, out_cap, wrap, cursor,
+                                        s_ws, &s_base);
+            }
+            __syncthreads();
+        }
     }
     const Acc r = block_sum(acc, s_red);
     if (threadIdx.x == 0) {
@@ -180,23 +360,29 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *s, int n) {
     }
 }
 
-// Loads n records into s as (key << 32 | payload), pads to pow2 with ~0.
-// Padding entries sort last; they equal a real (0xFFFFFFFF, 0xFFFFFFFF)
-// record bit for bit, so the first n sorted entries are exactly the tile.
-__device__ __forceinline__ int load_sorted(uint64_t *s, const Rec *__restrict__ src, int n) {
-    int np = 2;
-    while (np < n) np <<= 1;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-        uint64_t v = ~0ull;
-        if (i < n) {
-            const Rec r = src[i];
-            v = (uint64_t(r.x) << 32) | r.y;
+// Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload),
+// pads to a power of two with ~0 and sorts.  Padding entries sort last and
+// equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit for bit, so the first n
+// sorted entries are exactly the tile.
+__device__ __forceinline__ void gather_sorted(uint64_t *s, const Rec *__restrict__ part,
+                                              const PartView &v, int side, uint32_t t0, int n) {
+    int np2 = 2;
+    while (np2 < n) np2 <<= 1;
+    for (int i0 = 0; i0 < np2; i0 += kJoinThreads * kBatch) {
+        Rec r[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int i = i0 + k * kJoinThreads + threadIdx.x;
+            if (i < n) r[k] = part[v.index(side, t0 + i)];
         }
-        s[i] = v;
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int i = i0 + k * kJoinThreads + threadIdx.x;
+            if (i < np2) s[i] = i < n ? ((uint64_t(r[k].x) << 32) | r[k].y) : ~0ull;
+        }
     }
     __syncthreads();
-    bitonic_sort(s, np);
-    return np;
+    bitonic_sort(s, np2);
 }
 
 __device__ __forceinline__ int lower_bound_key(const uint64_t *s, int n, uint32_t k) {
@@ -226,28 +412,30 @@ __device__ __forceinline__ int upper_bound_key(const uint64_t *s, int lo, int n,
 // GROUPED: count + C1 via run closed forms (C2 not produced).
 template <bool C2, bool WRITE, bool GROUPED>
 __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
-    const Rec *__restrict__ part, const uint32_t *__restrict__ hist, int logF, int ncL, int ncR,
-    uint64_t nL, uint64_t nR, const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
+    const Rec *__restrict__ part, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR,
+    uint64_t nL, const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
     uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out, uint64_t out_cap,
     int wrap, unsigned long long *__restrict__ cursor) {
     __shared__ uint64_t s_b[kSmjCap];
     __shared__ uint64_t s_p[kSmjCap];
     __shared__ uint64_t s_pa[GROUPED ? kSmjCap + 1 : 1];
+    extern __shared__ uint32_t dyn[];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ unsigned long long s_base;
 
-    const int p = blockIdx.x;
-    const PartRange pr = part_range(hist, logF, ncL, ncR, nL, nR, p);
+    const int p = xcd_partition(blockIdx.x, 1 << logF);
+    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, nL, kSmjCap, s_ws);
+    const uint32_t nb = v.n[0], np = v.n[1];
     Acc acc;
     acc.zero();
 
-    for (uint64_t t0 = pr.lb; t0 < pr.le; t0 += kSmjCap) {
-        const int tn = int(min<uint64_t>(kSmjCap, pr.le - t0));
-        load_sorted(s_b, part + t0, tn);
+    for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
+        const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
+        gather_sorted(s_b, part, v, 0, t0, tn);
         if (GROUPED) {
-            // prefix sums of A over the sorted build tile (serial per chunk of
-            // 8 entries, then a block scan of chunk sums)
+            // prefix sums of A over the sorted build tile: each thread sums
+            // kPer consecutive entries, then a block scan of the sums
             constexpr int kPer = kSmjCap / kJoinThreads;
             uint64_t loc[kPer];
             uint64_t sum = 0;
@@ -258,7 +446,6 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
                 loc[q] = a;
                 sum += a;
             }
-            // exclusive scan of u64 sums across the block through LDS s_pa
             s_pa[threadIdx.x] = sum;
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -280,9 +467,9 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
             if (threadIdx.x == kJoinThreads - 1) s_pa[kSmjCap] = run;
             __syncthreads();
         }
-        for (uint64_t u0 = pr.rb; u0 < pr.re; u0 += kSmjCap) {
-            const int un = int(min<uint64_t>(kSmjCap, pr.re - u0));
-            load_sorted(s_p, part + u0, un);
+        for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
+            const int un = int(min<uint32_t>(kSmjCap, np - u0));
+            gather_sorted(s_p, part, v, 1, u0, un);
             if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
@@ -376,26 +563,24 @@ int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const 
     const int F = 1 << p.logF;
     const bool c2 = (p.flags & SJB_F_C2) != 0;
     const int bshift = p.kind == PART_HASH ? p.logF : 0;
+    const size_t dynb = part_view_bytes(p.ncL, p.ncR, kHjCap);
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+#define SJB_HJ(C2_, W_)                                                                           \
+    k_hj_join<C2_, W_><<<F, kJoinThreads, dynb, c->stream>>>(c->d_part, c->d_offs, p.logF, p.ncL,     \
+                                                          p.ncR, p.nL, bshift, L, R, c->d_partials, \
+                                                          out, out_cap, wrap, cur)
     if (p.payload == PAY_IDX) {
         if (c2)
-            k_hj_join<true, true><<<F, kJoinThreads, 0, c->stream>>>(
-                c->d_part, c->d_hist, p.logF, p.ncL, p.ncR, p.nL, p.nR, bshift, L, R, c->d_partials,
-                out, out_cap, wrap, cur);
+            SJB_HJ(true, true);
         else
-            k_hj_join<false, true><<<F, kJoinThreads, 0, c->stream>>>(
-                c->d_part, c->d_hist, p.logF, p.ncL, p.ncR, p.nL, p.nR, bshift, L, R, c->d_partials,
-                out, out_cap, wrap, cur);
+            SJB_HJ(false, true);
     } else {
         if (c2)
-            k_hj_join<true, false><<<F, kJoinThreads, 0, c->stream>>>(
-                c->d_part, c->d_hist, p.logF, p.ncL, p.ncR, p.nL, p.nR, bshift, L, R, c->d_partials,
-                out, out_cap, wrap, cur);
+            SJB_HJ(true, false);
         else
-            k_hj_join<false, false><<<F, kJoinThreads, 0, c->stream>>>(
-                c->d_part, c->d_hist, p.logF, p.ncL, p.ncR, p.nL, p.nR, bshift, L, R, c->d_partials,
-                out, out_cap, wrap, cur);
+            SJB_HJ(false, false);
     }
+#undef SJB_HJ
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -405,10 +590,11 @@ int launch_smj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const
     const int F = 1 << p.logF;
     const bool c2 = (p.flags & SJB_F_C2) != 0;
     const bool grouped = (p.flags & SJB_F_GROUPED) != 0 && p.payload == PAY_VAL;
+    const size_t dynb = part_view_bytes(p.ncL, p.ncR, kSmjCap);
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
-#define SJB_SMJ(C2_, W_, G_)                                                                      \
-    k_smj_join<C2_, W_, G_><<<F, kJoinThreads, 0, c->stream>>>(c->d_part, c->d_hist, p.logF,     \
-                                                               p.ncL, p.ncR, p.nL, p.nR, L, R,   \
+#define SJB_SMJ(C2_, W_, G_)                                                                     \
+    k_smj_join<C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(c->d_part, c->d_offs, p.logF,     \
+                                                               p.ncL, p.ncR, p.nL, L, R,         \
                                                                c->d_partials, out, out_cap, wrap, \
                                                                cur)
     if (p.payload == PAY_IDX) {

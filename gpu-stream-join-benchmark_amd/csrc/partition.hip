@@ -3,32 +3,20 @@
 // Replaces the reference's D-bucket counting sort (computeHistogram +
 // host prefix sum + buildHashTable, src/sjb/windowing/lazy/executor/GPU/
 // GPU_HJ.cu:9-39 and GPU_HJ_NoOutput.cu:53-76; smjHistogramKernel +
-// thrust::exclusive_scan + smjSortKernel, GPU_SMJ.cu:9-32) with a
-// fanout F = 2^logF that is independent of the key range:
-//   1. k_hist    one workgroup per 16384-tuple chunk builds an LDS
-//                histogram over F partitions and stores it partition-major
-//                (hist[p * nchunks + chunk]),
-//   2. k_scan_*  exclusive scan of that matrix on the device (no host round
-//                trip): entry (p, chunk) becomes the chunk's first slot in
-//                partition p,
-//   3. k_scatter re-reads the chunk and writes 8-byte records (key,
-//                payload) to their slots; ranks inside a chunk come from
-//                LDS atomics.
-// Both sides go through one launch of each kernel: blocks [0, ncL) take
-// left chunks and [ncL, ncL+ncR) right chunks, and the scan runs over the
-// concatenated matrix, so left records land in d_part[0, nL) and right
-// records in d_part[nL, nL+nR).
+// thrust::exclusive_scan + smjSortKernel, GPU_SMJ.cu:9-32) with a fanout
+// F = 2^logF that is independent of the key range, in ONE pass over the
+// input (k_partition below): each 16384-tuple chunk is sorted by partition
+// in LDS and written back contiguously as 8-byte (key, payload) records,
+// together with its F+1 local partition offsets.  No global histogram, no
+// scan, no host round trip.  Both sides go through one launch: blocks
+// [0, ncL) take left chunks and [ncL, ncL+ncR) right chunks; left records
+// land in d_part[0, nL), right records in d_part[nL, nL+nR).
 #include <algorithm>
 
 #include "sjb_device.hpp"
 #include "sjb_internal.hpp"
 
 namespace sjb {
-
-constexpr int kItems = kChunk / kPartThreads;  // 32 tuples per thread
-constexpr int kScanThreads = 512;
-constexpr int kScanItems = 8;
-constexpr int kScanSeg = kScanThreads * kScanItems;  // 4096 counters per block
 
 // Partition function of one key.  HASH: top logF bits of the Fibonacci hash
 // (bijective, so equal keys and only equal keys collide before truncation).
@@ -127,204 +115,159 @@ int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple
     int blocks = int(std::min<uint64_t>((n + kChunk - 1) / kChunk, 1024));
     if (blocks < 1) blocks = 1;
     // The scan-block buffer doubles as scratch for the partial min/max.
-    k_minmax_partial<<<blocks, kPartThreads, 0, c->stream>>>(L, nL, R, nR, c->d_scan_blocks);
+    k_minmax_partial<<<blocks, kPartThreads, 0, c->stream>>>(L, nL, R, nR, c->d_scratch);
     SJB_HIP_TRY(hipGetLastError());
-    k_minmax_final<<<1, 64, 0, c->stream>>>(c->d_scan_blocks, blocks, logF, c->d_pparams);
-    SJB_HIP_TRY(hipGetLastError());
-    return SJB_OK;
-}
-
-// ---------------------------------------------------------------------------
-// histogram
-// ---------------------------------------------------------------------------
-template <int KIND>
-__global__ __launch_bounds__(kPartThreads) void k_hist(
-    const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, int ncR, int logF, const PartParams *__restrict__ pp,
-    const RmiParams *__restrict__ rmi, uint32_t *__restrict__ hist) {
-    extern __shared__ uint32_t lds_hist[];
-    const int F = 1 << logF;
-    PartFn<KIND> part;
-    part.init(logF, pp, rmi);
-    const bool left = int(blockIdx.x) < ncL;
-    const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
-    const sjb_tuple_t *in = left ? L : R;
-    const uint64_t n = left ? nL : nR;
-    const int nc = left ? ncL : ncR;
-    uint32_t *out = left ? hist : hist + uint64_t(F) * ncL;
-
-    for (int p = threadIdx.x; p < F; p += kPartThreads) lds_hist[p] = 0;
-    __syncthreads();
-    const uint64_t base = uint64_t(chunk) * kChunk;
-#pragma unroll 8
-    for (int j = 0; j < kItems; j++) {
-        uint64_t i = base + uint64_t(j) * kPartThreads + threadIdx.x;
-        if (i < n) atomicAdd(&lds_hist[part(in[i].key)], 1u);
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < F; p += kPartThreads) out[uint64_t(p) * nc + chunk] = lds_hist[p];
-}
-
-// ---------------------------------------------------------------------------
-// exclusive scan of a u32 array (reduce -> scan of block sums -> apply)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ a,
-                                                              uint64_t n,
-                                                              uint32_t *__restrict__ block_sums) {
-    __shared__ uint32_t sw[kScanThreads / kWave];
-    const uint64_t base = uint64_t(blockIdx.x) * kScanSeg;
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; j++) {
-        uint64_t i = base + uint64_t(j) * kScanThreads + threadIdx.x;
-        if (i < n) s += a[i];
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) sw[threadIdx.x / kWave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int w = 0; w < kScanThreads / kWave; w++) t += sw[w];
-        block_sums[blockIdx.x] = t;
-    }
-}
-
-// Single workgroup: exclusive scan of nb block sums in place.
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ bs, int nb) {
-    __shared__ uint32_t wsum[1024 / kWave];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        int i = base + threadIdx.x;
-        uint32_t v = i < nb ? bs[i] : 0;
-        uint32_t tot;
-        uint32_t ex = wave_excl_scan(v, &tot);
-        if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = tot;
-        __syncthreads();
-        uint32_t wpre = 0, all = 0;
-        for (int w = 0; w < 1024 / kWave; w++) {
-            if (w < int(threadIdx.x / kWave)) wpre += wsum[w];
-            all += wsum[w];
-        }
-        if (i < nb) bs[i] = carry + wpre + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += all;
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t *__restrict__ a, uint64_t n,
-                                                             const uint32_t *__restrict__ block_sums) {
-    __shared__ uint32_t wsum[kScanThreads / kWave];
-    const uint64_t base = uint64_t(blockIdx.x) * kScanSeg;
-    // Each thread owns kScanItems consecutive counters.
-    uint32_t v[kScanItems];
-    uint32_t local = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; j++) {
-        uint64_t i = base + uint64_t(threadIdx.x) * kScanItems + j;
-        v[j] = i < n ? a[i] : 0;
-        local += v[j];
-    }
-    uint32_t tot;
-    uint32_t ex = wave_excl_scan(local, &tot);
-    if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = tot;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (int w = 0; w < int(threadIdx.x / kWave); w++) wpre += wsum[w];
-    uint32_t run = block_sums[blockIdx.x] + wpre + ex;
-#pragma unroll
-    for (int j = 0; j < kScanItems; j++) {
-        uint64_t i = base + uint64_t(threadIdx.x) * kScanItems + j;
-        if (i < n) a[i] = run;
-        run += v[j];
-    }
-}
-
-static int scan_inplace(sjb_ctx *c, uint32_t *a, uint64_t n) {
-    int nb = int((n + kScanSeg - 1) / kScanSeg);
-    k_scan_reduce<<<nb, kScanThreads, 0, c->stream>>>(a, n, c->d_scan_blocks);
-    SJB_HIP_TRY(hipGetLastError());
-    k_scan_top<<<1, 1024, 0, c->stream>>>(c->d_scan_blocks, nb);
-    SJB_HIP_TRY(hipGetLastError());
-    k_scan_apply<<<nb, kScanThreads, 0, c->stream>>>(a, n, c->d_scan_blocks);
+    k_minmax_final<<<1, 64, 0, c->stream>>>(c->d_scratch, blocks, logF, c->d_pparams);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
 
 // ---------------------------------------------------------------------------
-// scatter
+// single-pass chunk-major partitioning
 // ---------------------------------------------------------------------------
+// One workgroup of kPartThreads threads owns one chunk of kChunk tuples of
+// one side.  It loads its tuples (16 per thread, coalesced), ranks them in
+// an LDS histogram over F partitions, scans the histogram, places the 8-byte
+// records partition-sorted into an LDS staging buffer and writes the whole
+// chunk back contiguously — every global write is a full, coalesced line.
+// The chunk's local partition offsets (F+1 entries) go to
+// offs[chunk_id * (F+1) + p]; chunk_id runs over left chunks then right
+// chunks.  The join gathers partition p as the run
+// [offs[c][p], offs[c][p+1]) of every chunk c.
 template <int KIND, int PAY>
-__global__ __launch_bounds__(kPartThreads) void k_scatter(
+__global__ __launch_bounds__(kPartThreads) void k_partition(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, int ncR, int logF, const PartParams *__restrict__ pp,
-    const RmiParams *__restrict__ rmi, const uint32_t *__restrict__ hist, Rec *__restrict__ part_out) {
-    extern __shared__ uint32_t lds_cur[];
+    uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    extern __shared__ uint32_t lds[];
     const int F = 1 << logF;
+    uint32_t *hist = lds;                                  // F + 1 counters
+    uint32_t *ws = lds + F + 1;                            // wave sums
+    Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 1 + kPartThreads / kWave + 1) & ~1));
     PartFn<KIND> part;
     part.init(logF, pp, rmi);
     const bool left = int(blockIdx.x) < ncL;
     const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
     const sjb_tuple_t *in = left ? L : R;
     const uint64_t n = left ? nL : nR;
-    const int nc = left ? ncL : ncR;
-    const uint32_t *h = left ? hist : hist + uint64_t(F) * ncL;
-
-    for (int p = threadIdx.x; p < F; p += kPartThreads) lds_cur[p] = h[uint64_t(p) * nc + chunk];
-    __syncthreads();
     const uint64_t base = uint64_t(chunk) * kChunk;
-#pragma unroll 8
-    for (int j = 0; j < kItems; j++) {
-        uint64_t i = base + uint64_t(j) * kPartThreads + threadIdx.x;
-        if (i < n) {
-            const uint2 kv = *reinterpret_cast<const uint2 *>(&in[i]);
-            uint32_t pos = atomicAdd(&lds_cur[part(kv.x)], 1u);
-            part_out[pos] = make_uint2(kv.x, PAY == PAY_VAL ? kv.y : uint32_t(i));
+    const int len = int(min<uint64_t>(kChunk, n - base));
+    Rec *out = part_out + (left ? 0 : nL) + base;
+
+    for (int q = threadIdx.x; q <= F; q += kPartThreads) hist[q] = 0;
+    __syncthreads();
+    constexpr int kPer = kChunk / kPartThreads;
+    Rec rec[kPer];
+    uint32_t pid[kPer], rank[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = j * kPartThreads + threadIdx.x;
+        if (e < len) {
+            const uint2 kv = *reinterpret_cast<const uint2 *>(&in[base + e]);
+            rec[j] = make_uint2(kv.x, PAY == PAY_VAL ? kv.y : uint32_t(base + e));
         }
     }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = j * kPartThreads + threadIdx.x;
+        if (e < len) {
+            pid[j] = part(rec[j].x);
+            rank[j] = atomicAdd(&hist[pid[j]], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of hist[0..F) in place; hist[F] = len
+    {
+        constexpr int kMaxPer = (1 << kMaxLogF) / kPartThreads;
+        uint32_t v[kMaxPer];
+        uint32_t sum = 0;
+        const int per = (F + kPartThreads - 1) / kPartThreads;
+        for (int q = 0; q < per && q < kMaxPer; q++) {
+            const int idx = threadIdx.x * per + q;
+            v[q] = idx < F ? hist[idx] : 0;
+            sum += v[q];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan_u32(sum, ws, &tot);
+        for (int q = 0; q < per && q < kMaxPer; q++) {
+            const int idx = threadIdx.x * per + q;
+            if (idx < F) hist[idx] = run;
+            run += v[q];
+        }
+        if (threadIdx.x == 0) hist[F] = tot;
+    }
+    __syncthreads();
+    uint32_t *o = offs + uint64_t(blockIdx.x) * (F + 1);
+    for (int q = threadIdx.x; q <= F; q += kPartThreads) o[q] = hist[q];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = j * kPartThreads + threadIdx.x;
+        if (e < len) stage[hist[pid[j]] + rank[j]] = rec[j];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < len; e += kPartThreads) out[e] = stage[e];
 }
 
 int choose_logF(int alg, uint64_t n_build, int requested) {
     if (requested > 0) return std::min(std::max(requested, 1), kMaxLogF);
-    // Aim for half-full LDS tiles: expected partition = cap / 2.
-    const uint64_t target = alg == SJB_ALG_SMJ ? kSmjCap / 2 : kHjCap / 2;
+    // expected partition size at most ~0.92 (HJ) / 0.5 (SMJ) of an LDS tile,
+    // so uniform keys never spill into a second tile
+    const uint64_t target = alg == SJB_ALG_SMJ ? kSmjCap / 2 : (kHjCap * 23) / 25;
     int logF = 1;
     while (logF < kMaxLogF && (n_build >> logF) > target) logF++;
     return logF;
 }
 
+size_t partition_lds_bytes(int logF) {
+    const size_t words = ((size_t(1) << logF) + 1 + kPartThreads / kWave + 1) & ~size_t(1);
+    return words * 4 + size_t(kChunk) * sizeof(Rec);
+}
+
 template <int KIND>
 static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
                           const sjb_tuple_t *R) {
-    const int F = 1 << p.logF;
-    const size_t lds = size_t(F) * sizeof(uint32_t);
+    const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;
-    k_hist<KIND><<<nblk, kPartThreads, lds, c->stream>>>(L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF,
-                                                         c->d_pparams, c->d_rmi, c->d_hist);
-    SJB_HIP_TRY(hipGetLastError());
-    int rc = scan_inplace(c, c->d_hist, uint64_t(F) * nblk);
-    if (rc) return rc;
     if (p.payload == PAY_VAL)
-        k_scatter<KIND, PAY_VAL><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, c->d_hist, c->d_part);
+        k_partition<KIND, PAY_VAL><<<nblk, kPartThreads, lds, c->stream>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part);
     else
-        k_scatter<KIND, PAY_IDX><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, c->d_hist, c->d_part);
+        k_partition<KIND, PAY_IDX><<<nblk, kPartThreads, lds, c->stream>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
 
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R) {
-    if (uint64_t(1u << p.logF) * uint64_t(p.ncL + p.ncR) > c->hist_cap) return SJB_E_CAPACITY;
+    if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
+    if (uint64_t((1u << p.logF) + 1) * uint64_t(p.ncL + p.ncR) > c->offs_cap) return SJB_E_CAPACITY;
     switch (p.kind) {
         case PART_HASH: return partition_kind<PART_HASH>(c, p, L, R);
         case PART_RANGE: return partition_kind<PART_RANGE>(c, p, L, R);
         case PART_LEARNED: return partition_kind<PART_LEARNED>(c, p, L, R);
     }
     return SJB_E_INVALID;
+}
+
+// Raises the dynamic-LDS limit of the partition kernels (the staging buffer
+// is 128 KiB); called once per context.
+int partition_setup(sjb_ctx *c) {
+    const int need = int(partition_lds_bytes(kMaxLogF));
+    (void)c;
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_HASH, PAY_VAL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_HASH, PAY_IDX>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_RANGE, PAY_VAL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_RANGE, PAY_IDX>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_LEARNED, PAY_VAL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipFuncSetAttribute((const void *)k_partition<PART_LEARNED, PAY_IDX>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    (void)hipGetLastError();
+    return SJB_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -25,10 +25,11 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 typedef uint2 Rec;
 
 constexpr int kChunk = 16384;        // tuples per partition-pass workgroup
-constexpr int kPartThreads = 512;    // threads of the partition passes
-constexpr int kMaxLogF = 13;         // fanout bound: 8192 partitions
-constexpr int kJoinThreads = 256;    // threads of the per-partition join
-constexpr int kHjCap = 4096;         // build entries per LDS tile (HJ)
+constexpr int kPartThreads = 1024;   // threads of the partition pass (16 tuples each)
+constexpr int kMaxLogF = 12;         // fanout bound: 4096 partitions
+constexpr int kMaxChunks = 1024;     // chunks per side (windows up to 2^24 tuples)
+constexpr int kJoinThreads = 512;    // threads of the per-partition join
+constexpr int kHjCap = 3072;         // build entries per LDS tile (HJ)
 constexpr int kSmjCap = 2048;        // elements per LDS sort tile (SMJ)
 constexpr int kMaxHeavy = 32;        // heavy-key slots (grouped mode)
 
@@ -78,9 +79,9 @@ struct sjb_ctx {
     uint64_t cap = 0;                 // per-side tuple capacity
     sjb_tuple_t *d_stage[2] = {};     // H2D gather target of sjb_lazy_execute
     sjb::Rec *d_part = nullptr;       // partitioned records, left then right
-    uint32_t *d_hist = nullptr;       // F * (ncL + ncR) counters, scanned in place
-    uint32_t *d_scan_blocks = nullptr;
-    uint64_t hist_cap = 0;
+    uint32_t *d_offs = nullptr;       // (F+1) local partition offsets per chunk
+    uint64_t offs_cap = 0;
+    uint32_t *d_scratch = nullptr;    // small per-call scratch (min/max partials)
     uint64_t *d_partials = nullptr;   // 4 u64 per partition
     sjb_window_result_t *d_result = nullptr;
     sjb_window_result_t *h_result = nullptr;  // pinned
@@ -126,6 +127,7 @@ int choose_logF(int alg, uint64_t n_build, int requested);
 int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
                   int logF);
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
+int partition_setup(sjb_ctx *c);
 int launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, uint64_t n);
 int launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t v, uint64_t n);
 

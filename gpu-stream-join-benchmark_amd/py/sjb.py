@@ -106,6 +106,8 @@ def ptr(a):
     """Raw address of a numpy array or a torch tensor."""
     if a is None:
         return None
+    if isinstance(a, int):
+        return a
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     return a.ctypes.data
