@@ -139,7 +139,8 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, 1 << p.logF, d_res);
+    rc = launch_reduce(c, alg == SJB_ALG_HJ ? (1 << p.logF) * (kJoinThreads / kWave) : 1 << p.logF,
+                       d_res);
     if (rc) return rc;
     if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
     return SJB_OK;

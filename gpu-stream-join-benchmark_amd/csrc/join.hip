@@ -38,10 +38,7 @@
 
 namespace sjb {
 
-constexpr int kHjLogNB = 11;
 constexpr int kHjNB = 1 << kHjLogNB;
-constexpr uint32_t kNil32 = 0xFFFFFFFFu;
-constexpr uint16_t kNil16 = 0xFFFFu;
 constexpr int kBatch = 8;
 constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
 
@@ -102,28 +99,43 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     v.runid[1] = v.runid[0] + idcap;
     v.idcap = idcap;
     const int F1 = (1 << logF) + 1;
-    for (int side = 0; side < 2; side++) {
-        const int nc = v.nc[side];
-        const uint32_t *o = offs + uint64_t(side ? ncL : 0) * F1;
-        const uint64_t rec_base = side ? nL : 0;
-        uint32_t carry = 0;
-        for (int c0 = 0; c0 < nc; c0 += blockDim.x) {
-            const int c = c0 + threadIdx.x;
-            uint32_t len = 0;
-            if (c < nc) {
-                const uint32_t a = o[uint64_t(c) * F1 + p];
-                const uint32_t b = o[uint64_t(c) * F1 + p + 1];
-                v.start[side][c] = uint32_t(rec_base + uint64_t(c) * kChunk + a);
-                len = b - a;
-            }
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan_u32(len, ws, &tot);
-            if (c < nc) v.pref[side][c] = carry + ex;
-            carry += tot;
+    // Run bounds of both sides: slot q < ncL is left chunk q, slot ncL + c is
+    // right chunk c; one pass of global reads and one scan when they fit.
+    const int nq = ncL + ncR;
+    uint32_t carry = 0;
+    for (int q0 = 0; q0 < nq; q0 += blockDim.x) {
+        const int q = q0 + threadIdx.x;
+        uint32_t len = 0;
+        if (q < nq) {
+            const bool right = q >= ncL;
+            const int c = right ? q - ncL : q;
+            const uint32_t *o = offs + uint64_t(q) * F1;
+            const uint32_t a = o[p], b = o[p + 1];
+            v.start[right][c] = uint32_t((right ? nL : 0) + uint64_t(c) * kChunk + a);
+            len = b - a;
         }
-        if (threadIdx.x == 0) v.pref[side][nc] = carry;
-        v.n[side] = carry;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_u32(len, ws, &tot);
+        if (q < nq) {
+            const bool right = q >= ncL;
+            // right prefixes are rebased below once the left total is known
+            if (right)
+                v.pref[1][q - ncL] = carry + ex;
+            else
+                v.pref[0][q] = carry + ex;
+        }
+        carry += tot;
     }
+    __syncthreads();
+    const uint32_t nleft = ncL ? (ncR ? v.pref[1][0] : carry) : 0;
+    __syncthreads();  // every thread has read pref[1][0] before it is rebased
+    for (int c = threadIdx.x; c < ncR; c += blockDim.x) v.pref[1][c] -= nleft;
+    if (threadIdx.x == 0) {
+        v.pref[0][ncL] = nleft;
+        v.pref[1][ncR] = carry - nleft;
+    }
+    v.n[0] = nleft;
+    v.n[1] = carry - nleft;
     __syncthreads();
     for (int side = 0; side < 2; side++) {
         for (int c = threadIdx.x; c < v.nc[side]; c += blockDim.x) {
@@ -166,22 +178,66 @@ __device__ __forceinline__ int xcd_partition(int bid, int F) {
     return (bid % kXcd) * (F / kXcd) + bid / kXcd;
 }
 
-// LDS chained table of one build tile.
+// LDS table of one build tile in CSR form: records sorted by bucket
+// (bucket = hash bits below the partition bits), off[b] .. off[b+1] is
+// bucket b.  Built by an LDS counting sort; a probe reads two bounds and
+// scans contiguous keys (independent loads, no pointer chasing).
 struct HjTable {
     uint32_t key[kHjCap];
     uint32_t pay[kHjCap];
-    uint16_t next[kHjCap];
-    uint32_t head[kHjNB];
+    uint32_t off[kHjNB + 1];
 };
 
 __device__ __forceinline__ uint32_t hj_bucket(uint32_t k, int bshift) {
     return (key_hash(k) << bshift) >> (32 - kHjLogNB);
 }
 
-__device__ __forceinline__ void hj_insert(HjTable &t, uint32_t i, Rec r, int bshift) {
-    t.key[i] = r.x;
-    t.pay[i] = r.y;
-    t.next[i] = uint16_t(atomicExch(&t.head[hj_bucket(r.x, bshift)], i));
+// Builds the CSR table from up to kHjPerT register-resident records per
+// thread (element i = k * blockDim + tid < tn).  Starts and ends with a
+// barrier; off[] must be zero on entry.
+__device__ __forceinline__ void hj_build(HjTable &t, const Rec (&r)[kHjPerT], uint32_t tn, int bshift,
+                                         uint32_t *ws) {
+    uint32_t slot[kHjPerT];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kHjPerT; k++) {
+        const uint32_t i = k * kJoinThreads + threadIdx.x;
+        if (i < tn) {
+            const uint32_t b = hj_bucket(r[k].x, bshift);
+            slot[k] = (b << 16) | atomicAdd(&t.off[b], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the kHjNB bucket counts
+    constexpr int kPerB = (kHjNB + kJoinThreads - 1) / kJoinThreads;
+    uint32_t v[kPerB];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPerB; q++) {
+        const int idx = threadIdx.x * kPerB + q;
+        v[q] = idx < kHjNB ? t.off[idx] : 0;
+        sum += v[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan_u32(sum, ws, &tot);
+#pragma unroll
+    for (int q = 0; q < kPerB; q++) {
+        const int idx = threadIdx.x * kPerB + q;
+        if (idx < kHjNB) t.off[idx] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) t.off[kHjNB] = tot;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kHjPerT; k++) {
+        const uint32_t i = k * kJoinThreads + threadIdx.x;
+        if (i < tn) {
+            const uint32_t pos = t.off[slot[k] >> 16] + (slot[k] & 0xFFFFu);
+            t.key[pos] = r[k].x;
+            t.pay[pos] = r[k].y;
+        }
+    }
+    __syncthreads();
 }
 
 // Gathers elements [base, base+n) of one side of the partition, kBatch
@@ -204,17 +260,38 @@ __device__ __forceinline__ void gather(const Rec *__restrict__ part, const PartV
     }
 }
 
-// count / C1 / C2 of one probe record (payload = val) against the tile
+template <bool C2>
+__device__ __forceinline__ void pair_acc(uint32_t k, uint32_t rv, uint32_t sv, Acc &acc) {
+    const uint64_t a = hash_a(k, rv);
+    acc.c1 += a * hash_b(k, sv);
+    if (C2) acc.c2 += pair_hash_from_a(a, sv);
+}
+
+// count / C1 / C2 of one probe record (payload = val).  The bucket scan
+// only compares keys and keeps the first two matching build payloads in
+// registers; the 64-bit checksum math runs once per probe after the scan
+// (a wave executes it once for all its matching lanes instead of once per
+// step).  Later matches in one bucket are handled inline.
 template <bool C2>
 __device__ __forceinline__ void hj_probe_val(const HjTable &t, Rec s, int bshift, Acc &acc) {
-    for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e]) {
+    const uint32_t b = hj_bucket(s.x, bshift);
+    const uint32_t lo = t.off[b], hi = t.off[b + 1];
+    uint32_t m = 0, rv0 = 0, rv1 = 0;
+    for (uint32_t e = lo; e < hi; e++) {
         if (t.key[e] == s.x) {
-            const uint64_t a = hash_a(s.x, t.pay[e]);
-            acc.count++;
-            acc.c1 += a * hash_b(s.x, s.y);
-            if (C2) acc.c2 += pair_hash_from_a(a, s.y);
+            const uint32_t rv = t.pay[e];
+            if (m == 0)
+                rv0 = rv;
+            else if (m == 1)
+                rv1 = rv;
+            else
+                pair_acc<C2>(s.x, rv, s.y, acc);
+            m++;
         }
     }
+    acc.count += m;
+    if (m >= 1) pair_acc<C2>(s.x, rv0, s.y, acc);
+    if (m >= 2) pair_acc<C2>(s.x, rv1, s.y, acc);
 }
 
 // Probes the un records staged in LDS against the current tile.  WRITE
@@ -238,8 +315,8 @@ __device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, ui
         uint32_t m = 0;
         if (j < un) {
             s = sp[j];
-            for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e])
-                m += t.key[e] == s.x;
+            const uint32_t b = hj_bucket(s.x, bshift);
+            for (uint32_t e = t.off[b]; e < t.off[b + 1]; e++) m += t.key[e] == s.x;
         }
         uint32_t tot;
         const uint32_t ex = block_excl_scan_u32(m, ws, &tot);
@@ -248,7 +325,8 @@ __device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, ui
         if (m) {
             uint64_t pos = *s_base + ex;
             const sjb_tuple_t st = R[s.y];
-            for (uint32_t e = t.head[hj_bucket(s.x, bshift)]; e < kNil16; e = t.next[e]) {
+            const uint32_t b = hj_bucket(s.x, bshift);
+            for (uint32_t e = t.off[b]; e < t.off[b + 1]; e++) {
                 if (t.key[e] != s.x) continue;
                 const sjb_tuple_t lt = L[t.pay[e]];
                 const uint64_t a = hash_a(lt.key, lt.val);
@@ -274,21 +352,21 @@ __global__ __launch_bounds__(kJoinThreads) void k_hj_join(
     int wrap, unsigned long long *__restrict__ cursor) {
     __shared__ HjTable t;
     __shared__ Rec s_probe[kHjCap];
-    __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ unsigned long long s_base;
     extern __shared__ uint32_t dyn[];
 
     const int p = xcd_partition(blockIdx.x, 1 << logF);
-    for (int i = threadIdx.x; i < kHjNB; i += kJoinThreads) t.head[i] = kNil32;
+    for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
     const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, nL, kHjCap, s_ws);
     const uint32_t nb = v.n[0], np = v.n[1];
     Acc acc;
     acc.zero();
     if (nb && np) {
         const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjCap);
+        Rec rb[kHjPerT];
         {
-            Rec rb[kHjPerT], rp[kHjPerT];
+            Rec rp[kHjPerT];
 #pragma unroll
             for (int k = 0; k < kHjPerT; k++) {
                 const uint32_t i = k * kJoinThreads + threadIdx.x;
@@ -301,21 +379,23 @@ __global__ __launch_bounds__(kJoinThreads) void k_hj_join(
             }
 #pragma unroll
             for (int k = 0; k < kHjPerT; k++) {
-                const uint32_t i = k * kJoinThreads + threadIdx.x;
-                if (i < tn0) hj_insert(t, i, rb[k], bshift);
-                if (i < pn0) s_probe[i] = rp[k];
+                const uint32_t j = k * kJoinThreads + threadIdx.x;
+                if (j < pn0) s_probe[j] = rp[k];
             }
         }
-        __syncthreads();
         uint32_t resident = 0;  // first element of the probe chunk in s_probe
         for (uint32_t t0 = 0; t0 < nb; t0 += kHjCap) {
+            const uint32_t tn = min<uint32_t>(kHjCap, nb - t0);
             if (t0 > 0) {
-                for (int i = threadIdx.x; i < kHjNB; i += kJoinThreads) t.head[i] = kNil32;
                 __syncthreads();
-                gather(part, v, 0, t0, min<uint32_t>(kHjCap, nb - t0),
-                       [&](uint32_t i, Rec r) { hj_insert(t, i, r, bshift); });
-                __syncthreads();
+                for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
+#pragma unroll
+                for (int k = 0; k < kHjPerT; k++) {
+                    const uint32_t i = k * kJoinThreads + threadIdx.x;
+                    if (i < tn) rb[k] = part[v.index(0, t0 + i)];
+                }
             }
+            hj_build(t, rb, tn, bshift, s_ws);
             for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
                 const uint32_t un = min<uint32_t>(kHjCap, np - u0);
                 if (u0 != resident) {
@@ -330,11 +410,15 @@ __global__ __launch_bounds__(kJoinThreads) void k_hj_join(
             __syncthreads();
         }
     }
-    const Acc r = block_sum(acc, s_red);
-    if (threadIdx.x == 0) {
-        partials[3 * p] = r.count;
-        partials[3 * p + 1] = r.c1;
-        partials[3 * p + 2] = r.c2;
+    // one partial per wave (summed by k_reduce): no block-level reduction
+    acc.count = wave_sum(acc.count);
+    acc.c1 = wave_sum(acc.c1);
+    acc.c2 = wave_sum(acc.c2);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const uint64_t w = uint64_t(p) * (kJoinThreads / kWave) + threadIdx.x / kWave;
+        partials[3 * w] = acc.count;
+        partials[3 * w + 1] = acc.c1;
+        partials[3 * w + 2] = acc.c2;
     }
 }
 
@@ -433,7 +517,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
         gather_sorted(s_b, part, v, 0, t0, tn);
-        if (GROUPED) {
+        if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
             constexpr int kPer = kSmjCap / kJoinThreads;
@@ -476,7 +560,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
                     const int lo = lower_bound_key(s_b, tn, k);
                     const int hi = upper_bound_key(s_b, lo, tn, k);
                     if (hi == lo) continue;
-                    if (GROUPED) {
+                    if constexpr (GROUPED) {
                         acc.count += uint64_t(hi - lo);
                         acc.c1 += (s_pa[hi] - s_pa[lo]) * hash_b(k, sv);
                     } else {
@@ -533,15 +617,30 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
 // ---------------------------------------------------------------------------
 // partial reduction -> one sjb_window_result_t
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce(const uint64_t *__restrict__ partials, int n,
-                                                sjb_window_result_t *__restrict__ out) {
-    __shared__ uint64_t s_red[3 * 256 / kWave];
+__global__ __launch_bounds__(1024) void k_reduce(const uint64_t *__restrict__ partials, int n,
+                                                 sjb_window_result_t *__restrict__ out) {
+    __shared__ uint64_t s_red[3 * 1024 / kWave];
     Acc a;
     a.zero();
-    for (int i = threadIdx.x; i < n; i += 256) {
-        a.count += partials[3 * i];
-        a.c1 += partials[3 * i + 1];
-        a.c2 += partials[3 * i + 2];
+    constexpr int kU = 8;
+    for (int i0 = 0; i0 < n; i0 += 1024 * kU) {
+        uint64_t v[kU][3];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int i = i0 + u * 1024 + threadIdx.x;
+            v[u][0] = v[u][1] = v[u][2] = 0;
+            if (i < n) {
+                v[u][0] = partials[3 * i];
+                v[u][1] = partials[3 * i + 1];
+                v[u][2] = partials[3 * i + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            a.count += v[u][0];
+            a.c1 += v[u][1];
+            a.c2 += v[u][2];
+        }
     }
     const Acc r = block_sum(a, s_red);
     if (threadIdx.x == 0) {
@@ -553,7 +652,7 @@ __global__ __launch_bounds__(256) void k_reduce(const uint64_t *__restrict__ par
 }
 
 int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out) {
-    k_reduce<<<1, 256, 0, c->stream>>>(c->d_partials, n_partials, d_out);
+    k_reduce<<<1, 1024, 0, c->stream>>>(c->d_partials, n_partials, d_out);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }

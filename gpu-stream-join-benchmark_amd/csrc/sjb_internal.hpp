@@ -24,13 +24,33 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // count/checksum work, or its index in the window for pair output).
 typedef uint2 Rec;
 
-constexpr int kChunk = 16384;        // tuples per partition-pass workgroup
-constexpr int kPartThreads = 1024;   // threads of the partition pass (16 tuples each)
-constexpr int kMaxLogF = 12;         // fanout bound: 4096 partitions
-constexpr int kMaxChunks = 1024;     // chunks per side (windows up to 2^24 tuples)
-constexpr int kJoinThreads = 512;    // threads of the per-partition join
-constexpr int kHjCap = 3072;         // build entries per LDS tile (HJ)
-constexpr int kSmjCap = 2048;        // elements per LDS sort tile (SMJ)
+// Tuning constants (overridable at build time with -DSJB_*).
+#ifndef SJB_CHUNK
+#define SJB_CHUNK 16384
+#endif
+#ifndef SJB_PART_THREADS
+#define SJB_PART_THREADS 1024
+#endif
+#ifndef SJB_JOIN_THREADS
+#define SJB_JOIN_THREADS 512
+#endif
+#ifndef SJB_HJ_CAP
+#define SJB_HJ_CAP 3072
+#endif
+#ifndef SJB_HJ_LOGNB
+#define SJB_HJ_LOGNB 11
+#endif
+#ifndef SJB_SMJ_CAP
+#define SJB_SMJ_CAP 2048
+#endif
+constexpr int kChunk = SJB_CHUNK;              // tuples per partition-pass workgroup
+constexpr int kPartThreads = SJB_PART_THREADS; // threads of the partition pass
+constexpr int kMaxLogF = 12;                   // fanout bound: 4096 partitions
+constexpr int kMaxChunks = 1024;               // chunks per side
+constexpr int kJoinThreads = SJB_JOIN_THREADS; // threads of the per-partition join
+constexpr int kHjCap = SJB_HJ_CAP;             // build entries per LDS tile (HJ)
+constexpr int kHjLogNB = SJB_HJ_LOGNB;         // log2 LDS hash buckets (HJ)
+constexpr int kSmjCap = SJB_SMJ_CAP;           // elements per LDS sort tile (SMJ)
 constexpr int kMaxHeavy = 32;        // heavy-key slots (grouped mode)
 
 enum PartKind : int { PART_HASH = 0, PART_RANGE = 1, PART_LEARNED = 2 };

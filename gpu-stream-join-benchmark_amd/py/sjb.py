@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG, "lib", "libsjb_hip.so")
+LIB_PATH = os.environ.get("SJB_LIB") or os.path.join(PKG, "lib", "libsjb_hip.so")
 
 # enums (include/sjb/windowing/BaseWindowing.cuh:9-40 values)
 ALG = {"EMPTY": 0, "NLJ": 1, "HJ": 2, "SMJ": 3}
