@@ -341,6 +341,9 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
 int sjb_close(sjb_ctx_t *c) {
     if (!c) return SJB_E_INVALID;
     (void)hipSetDevice(c->device);
+    if (c->pk_ready) (void)sjb_pk_stop(c);
+    if (c->pk_stream) (void)hipStreamDestroy(c->pk_stream);
+    if (c->pk_copy) (void)hipStreamDestroy(c->pk_copy);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : c->dev_allocs) (void)hipFree(p);
     for (void *p : c->host_allocs) (void)hipHostFree(p);

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_eager_probe(
     }
 }
 
-static int eager_init(sjb_ctx *c) {
+int eager_init(sjb_ctx *c) {
     if (c->eager_ready) return SJB_OK;
     const uint64_t W = c->opts.max_window_tuples;
     if (W >= kEagerNil) return SJB_E_CAPACITY;
@@ -170,6 +170,7 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
     if (method != SJB_OUT_NOOUTPUT && out_cap > 0 && !out) return SJB_E_INVALID;
     if (method == SJB_OUT_ATOMIC && out_cap == 0) return SJB_E_INVALID;
     memset(stats, 0, sizeof(*stats));
+    if (c->pk_ready) return SJB_E_STATE;  // the window state belongs to the persistent kernel
     SJB_HIP_TRY(hipSetDevice(c->device));
     int rc = eager_init(c);
     if (rc) return rc;
@@ -197,6 +198,33 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
     }
     const uint64_t n_other = c->eager_n[other];
     sjb_window_result_t res{};
+    if (write && !wrap && n_other) {
+        // Count first so an undersized output buffer is rejected before the
+        // state changes (the caller can grow the buffer and call again).
+        if (algorithm == SJB_ALG_HJ) {
+            const unsigned blocks = unsigned((n + 255) / 256);
+            k_eager_probe<false, false><<<blocks, 256, 0, c->stream>>>(
+                c->d_batch, n, c->d_eager_tuples[other], c->d_eager_next[other], c->d_eager_head[other],
+                uint32_t(c->eager_nb - 1), is_left, c->d_partials, nullptr, 0, 0, cur);
+            SJB_HIP_TRY(hipGetLastError());
+            rc = launch_reduce(c, int(blocks), c->d_result);
+        } else {
+            const sjb_tuple_t *L0 = is_left ? c->d_batch : c->d_eager_tuples[other];
+            const sjb_tuple_t *R0 = is_left ? c->d_eager_tuples[other] : c->d_batch;
+            rc = launch_nlj(c, L0, is_left ? n : n_other, R0, is_left ? n_other : n, 0, 0, nullptr, 0, 0,
+                            c->d_result);
+        }
+        if (rc) return rc;
+        SJB_HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(sjb_window_result_t),
+                                   hipMemcpyDeviceToHost, c->stream));
+        SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->h_result->count > out_cap) {
+            stats->result_count = c->h_result->count;
+            return SJB_E_CAPACITY;
+        }
+        rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
+        if (rc) return rc;
+    }
     if (n_other == 0) {
         rc = launch_fill_u64(c->stream, reinterpret_cast<uint64_t *>(c->d_result), 0, 4);
         if (rc) return rc;

@@ -134,6 +134,22 @@ struct sjb_ctx {
     uint64_t eager_n[2] = {0, 0};
     sjb_tuple_t *d_batch = nullptr;   // staged incoming batch
     uint64_t batch_cap = 0;
+
+    // persistent-kernel eager executor (pk.hip)
+    hipStream_t pk_stream = nullptr;   // runs the persistent kernel
+    hipStream_t pk_copy = nullptr;     // batch copies + doorbell writes
+    void *d_pk_ctl = nullptr;          // doorbell / control words (device)
+    void *d_pk_descs = nullptr;        // work descriptors (device ring)
+    sjb_tuple_t *d_pk_slots = nullptr; // batch slots (device ring)
+    void *h_pk_res = nullptr;          // results + completion (coherent pinned host)
+    void *h_pk_stage = nullptr;        // pinned staging of descriptors
+    uint64_t pk_seq = 0;               // next sequence number
+    uint64_t pk_slot_cap = 0;          // tuples per batch slot
+    uint64_t pk_n[2] = {0, 0};         // host view of the state sizes
+    uint64_t pk_idle_limit = 0;        // polls before an idle kernel exits
+    uint64_t pk_launches = 0;
+    bool pk_ready = false;
+    bool pk_running = false;
 };
 
 namespace sjb {
@@ -188,5 +204,8 @@ struct OutBuf {
 };
 int out_prepare(sjb_ctx *c, sjb_result_tuple_t *out, uint64_t cap, OutBuf *ob);
 int out_finish(sjb_ctx *c, const OutBuf &ob, uint64_t rows);
+
+// ---- eager.hip ----
+int eager_init(sjb_ctx *c);
 
 }  // namespace sjb

@@ -52,6 +52,22 @@ public:
         : GPU_Eager(SJB_ALG_NLJ, SJB_OUT_BITMAP, 1000, windowSize, batchSize) {}
 };
 
+// Persistent-kernel eager hash join (the design of
+// src/sjb/windowing/eager/executor/GPU/GPU_HJ_PK.cu, GPU_HJ_PK.cuh:19-40):
+// batches go through sjb_pk_execute to one resident workgroup.
+class GPU_HJ_PK : public BaseEagerExecutor {
+public:
+    GPU_HJ_PK(uint64_t numDistinctKeys, uint64_t windowSize, uint64_t batchSize);
+    ~GPU_HJ_PK() override;
+    uint64_t execute(Tuple *tupleBuffer, bool isLeftSide, uint64_t windowTupleCount, Sink &sink) override;
+    void clearStates() override;
+    void onEndOfStream() override;
+
+private:
+    uint64_t batchSize;
+    sjb_ctx_t *ctx = nullptr;
+};
+
 // src/sjb/windowing/eager/executor/CPU/CPU_HJ.cpp: build own side, probe the other.
 class CPU_HJ : public BaseEagerExecutor {
 public:

@@ -1,0 +1,68 @@
+// Sink (src/sjb/sink/Sink.cpp:6-125).
+#include "sjb/sink/Sink.h"
+
+#include <algorithm>
+#include <cstring>
+
+Sink::Sink(uint64_t sinkBufferSize) : sinkBuffer(sinkBufferSize), sinkBufferSize(sinkBufferSize) {}
+Sink::~Sink() = default;
+
+void Sink::addToCounter(uint64_t currentMatches, bool isWindowClosed) {
+    const uint64_t now = counter.fetch_add(currentMatches) + currentMatches;
+    const auto t = std::chrono::high_resolution_clock::now();
+    const auto us = std::chrono::duration_cast<std::chrono::microseconds>(t - processingStartTime).count();
+    std::lock_guard<std::mutex> g(progressMutex);
+    progress.push_back(Progress{now, isWindowClosed, uint64_t(us)});
+}
+
+uint64_t Sink::getMatchCount(bool useGPUCounter) { return useGPUCounter ? counterGPU.load() : counter.load(); }
+const std::atomic<uint64_t> &Sink::getPossibleMatches() const { return possibleMatches; }
+void Sink::addToPossibleMatches(uint64_t c) { possibleMatches.fetch_add(c); }
+void Sink::addToBuildTime(uint64_t t) { totalBuildTime.fetch_add(t); }
+void Sink::addToProbeTime(uint64_t t) { totalProbeTime.fetch_add(t); }
+void Sink::addToSortTime(uint64_t t) { totalSortTime.fetch_add(t); }
+void Sink::addToMergeTime(uint64_t t) { totalMergeTime.fetch_add(t); }
+void Sink::addToExecutedWindows(uint64_t n) { totalExecutedWindows.fetch_add(n); }
+void Sink::addToChecksums(uint64_t c1, uint64_t c2) {
+    checksum1.fetch_add(c1);
+    checksum2.fetch_add(c2);
+}
+uint64_t Sink::getSinkBufferSize() const { return sinkBufferSize; }
+const std::atomic<uint64_t> &Sink::getTotalExecutedWindows() const { return totalExecutedWindows; }
+
+// The reference divides by the executed-window count even when it is 0
+// (src/sjb/sink/Sink.cpp:69-83); these return 0 then.
+static uint64_t avg(const std::atomic<uint64_t> &tot, const std::atomic<uint64_t> &n) {
+    const uint64_t d = n.load();
+    return d ? tot.load() / d : 0;
+}
+uint64_t Sink::getAverageSortTime() { return avg(totalSortTime, totalExecutedWindows); }
+uint64_t Sink::getAverageMergeTime() { return avg(totalMergeTime, totalExecutedWindows); }
+uint64_t Sink::getAverageBuildTime() { return avg(totalBuildTime, totalExecutedWindows); }
+uint64_t Sink::getAverageProbeTime() { return avg(totalProbeTime, totalExecutedWindows); }
+
+std::vector<Progress> Sink::getProgress() {
+    std::lock_guard<std::mutex> g(progressMutex);
+    return progress;
+}
+
+void Sink::markProcessingStart() { processingStartTime = std::chrono::high_resolution_clock::now(); }
+
+void Sink::incrementCounterAndStore(uint32_t lkey, uint32_t rkey, uint32_t lVal, uint32_t rVal, uint64_t lTs,
+                                    uint64_t rTs) {
+    const uint64_t pos = counter.fetch_add(1);
+    if (sinkBufferSize == 0) return;
+    ResultTuple &r = sinkBuffer[pos % sinkBufferSize];
+    r.lkey = lkey;
+    r.rkey = rkey;
+    r.lVal = lVal;
+    r.rVal = rVal;
+    r.lTs = lTs;
+    r.rTs = rTs;
+}
+
+void Sink::incrementCounterAndStore(ResultTuple *resultTuple, uint64_t tupleCount) {
+    counter.fetch_add(tupleCount);
+    const uint64_t n = std::min(tupleCount, sinkBufferSize);
+    if (n && resultTuple != sinkBuffer.data()) std::memcpy(sinkBuffer.data(), resultTuple, n * sizeof(ResultTuple));
+}

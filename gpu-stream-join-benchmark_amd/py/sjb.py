@@ -72,6 +72,10 @@ SIGNATURES = {
     "sjb_eager_execute": (_i, [_p, _i, _i, _u32, _p, _u64, _i, _p, _u64, ctypes.POINTER(Stats)]),
     "sjb_eager_clear": (_i, [_p]),
     "sjb_eager_end_of_stream": (_i, [_p]),
+    "sjb_pk_execute": (_i, [_p, _p, _u64, _i, ctypes.POINTER(Stats)]),
+    "sjb_pk_clear": (_i, [_p]),
+    "sjb_pk_stop": (_i, [_p]),
+    "sjb_pk_launches": (_u64, [_p]),
     "sjb_generate": (_i, [_p, _u64, _i, _u64, _d, _u64, _u64]),
     "sjb_mix64": (_u64, [_u64]),
     "sjb_hash_a": (_u64, [_u32, _u32]),
@@ -196,3 +200,17 @@ class Context:
 
     def eager_clear(self):
         check(lib().sjb_eager_clear(self.h), "sjb_eager_clear")
+
+    def pk_execute(self, batch, is_left):
+        st = Stats()
+        rc = lib().sjb_pk_execute(self.h, ptr(batch), len(batch), 1 if is_left else 0, ctypes.byref(st))
+        return rc, st
+
+    def pk_clear(self):
+        check(lib().sjb_pk_clear(self.h), "sjb_pk_clear")
+
+    def pk_stop(self):
+        check(lib().sjb_pk_stop(self.h), "sjb_pk_stop")
+
+    def pk_launches(self):
+        return lib().sjb_pk_launches(self.h)

@@ -199,6 +199,21 @@ int sjb_eager_execute(sjb_ctx_t *ctx, int algorithm, int method, uint32_t flags,
 int sjb_eager_clear(sjb_ctx_t *ctx);
 int sjb_eager_end_of_stream(sjb_ctx_t *ctx);
 
+/* ---- persistent-kernel eager join -------------------------------------- */
+/* The eager hash join of sjb_eager_execute (count/checksums) served by one
+ * resident workgroup that polls a device-memory work queue, so a batch costs
+ * one H2D copy and a doorbell write instead of kernel launches.  Replaces the
+ * design of GPU_HJ_PK (src/sjb/windowing/eager/executor/GPU/GPU_HJ_PK.cu:10-249,
+ * include/sjb/windowing/eager/executor/GPU/GPU_HJ_PK.cuh:19-40; not compiled
+ * in the reference).  Batches hold at most opts.batch_size tuples.  The
+ * kernel exits by itself when idle (environment SJB_PK_IDLE_POLLS polls) and
+ * is relaunched on the next batch; sjb_pk_stop ends it, sjb_close stops it. */
+int sjb_pk_execute(sjb_ctx_t *ctx, const sjb_tuple_t *batch, uint64_t n, int is_left, sjb_stats_t *stats);
+int sjb_pk_clear(sjb_ctx_t *ctx);
+int sjb_pk_stop(sjb_ctx_t *ctx);
+/* Number of persistent-kernel launches so far (relaunches after idle exits). */
+uint64_t sjb_pk_launches(sjb_ctx_t *ctx);
+
 /* ---- data generation (host, deterministic) ----------------------------- */
 /* Fills n tuples: key distribution by `dist` (0 uniform over [0,key_range),
  * 1 Zipf(theta) over [0,key_range) with ranks scrambled by a fixed
