@@ -120,7 +120,7 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     p.flags = flags;
     p.nL = nL;
     p.nR = nR;
-    p.logF = choose_logF(alg, nL, int(c->opts.fanout_log2));
+    p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2));
     p.kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
              : alg == SJB_ALG_SMJ                     ? PART_RANGE
                                                       : PART_HASH;
@@ -139,8 +139,7 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, alg == SJB_ALG_HJ ? (1 << p.logF) * (kJoinThreads / kWave) : 1 << p.logF,
-                       d_res);
+    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res);
     if (rc) return rc;
     if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
     return SJB_OK;
@@ -273,6 +272,7 @@ const char *sjb_strerror(int code) {
         case SJB_E_BUDGET: return "HBM budget exceeded";
         case SJB_E_TIMEOUT: return "bounded wait expired";
         case SJB_E_STATE: return "invalid state";
+        case SJB_E_MISMATCH: return "launch shapes gave different results";
     }
     return "unknown error";
 }
@@ -345,6 +345,8 @@ int sjb_close(sjb_ctx_t *c) {
     if (c->pk_stream) (void)hipStreamDestroy(c->pk_stream);
     if (c->pk_copy) (void)hipStreamDestroy(c->pk_copy);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    sjb::batch_plan_free(c->scratch_plan);
+    c->scratch_plan = nullptr;
     for (void *p : c->dev_allocs) (void)hipFree(p);
     for (void *p : c->host_allocs) (void)hipHostFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -41,6 +41,7 @@ namespace sjb {
 constexpr int kHjNB = 1 << kHjLogNB;
 constexpr int kBatch = 8;
 constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
+constexpr int kPartialsPerPart = kJoinThreads / kWave;  // single-window partials per partition
 
 // Record index of element j (< pref[nc]) by binary search over the run
 // prefix (used only past the run-id table).
@@ -84,10 +85,13 @@ __host__ __device__ constexpr size_t part_view_bytes(int ncL, int ncR, int idcap
 }
 
 // Builds the PartView of partition p in `dyn` (dynamic LDS); ends with a
-// barrier.
+// barrier.  `offs` is the window's first chunk-offsets row; left chunk c
+// starts at record lbase + c*chunk of d_part, right chunk c at
+// rbase + c*chunk.
 __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__restrict__ offs,
-                                              int logF, int p, int ncL, int ncR, uint64_t nL,
-                                              uint32_t idcap, uint32_t *ws) {
+                                              int logF, int p, int ncL, int ncR, uint32_t lbase,
+                                              uint32_t rbase, uint32_t chunk, uint32_t idcap,
+                                              uint32_t *ws) {
     PartView v;
     v.nc[0] = ncL;
     v.nc[1] = ncR;
@@ -111,7 +115,7 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
             const int c = right ? q - ncL : q;
             const uint32_t *o = offs + uint64_t(q) * F1;
             const uint32_t a = o[p], b = o[p + 1];
-            v.start[right][c] = uint32_t((right ? nL : 0) + uint64_t(c) * kChunk + a);
+            v.start[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a;
             len = b - a;
         }
         uint32_t tot;
@@ -340,86 +344,244 @@ __device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, ui
     }
 }
 
-// One workgroup per partition.  The first build tile and the first kHjCap
+// Arguments every per-partition join body needs.
+struct JoinIO {
+    const Rec *part;
+    const sjb_tuple_t *L, *R;             // window inputs (pair output only)
+    sjb_result_tuple_t *out;
+    uint64_t out_cap;
+    int wrap;
+    unsigned long long *cursor;
+};
+
+// LDS of the HJ body: the CSR build table and the probe staging buffer.
+struct HjLds {
+    HjTable t;
+    Rec probe[kHjCap];
+};
+
+// Joins one partition (HJ).  The first build tile and the first kHjCap
 // probe records are gathered with all loads in flight at once (build
 // records go straight into the LDS table, probe records into an LDS
-// staging buffer); larger partitions continue tile by tile.
+// staging buffer); larger partitions continue tile by tile.  The caller
+// has built `v` (load_view ends with a barrier).
 template <bool C2, bool WRITE>
-__global__ __launch_bounds__(kJoinThreads) void k_hj_join(
-    const Rec *__restrict__ part, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR,
-    uint64_t nL, int bshift, const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
-    uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out, uint64_t out_cap,
-    int wrap, unsigned long long *__restrict__ cursor) {
-    __shared__ HjTable t;
-    __shared__ Rec s_probe[kHjCap];
-    __shared__ uint32_t s_ws[kJoinThreads / kWave];
-    __shared__ unsigned long long s_base;
-    extern __shared__ uint32_t dyn[];
-
-    const int p = xcd_partition(blockIdx.x, 1 << logF);
-    for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
-    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, nL, kHjCap, s_ws);
+__device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v, int bshift,
+                                             HjLds &m, uint32_t *s_ws, unsigned long long *s_base,
+                                             Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
-    Acc acc;
-    acc.zero();
-    if (nb && np) {
-        const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjCap);
-        Rec rb[kHjPerT];
-        {
-            Rec rp[kHjPerT];
+    if (!nb || !np) return;
+    HjTable &t = m.t;
+    Rec *s_probe = m.probe;
+    const Rec *__restrict__ part = io.part;
+    for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
+    const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjCap);
+    Rec rb[kHjPerT];
+    {
+        Rec rp[kHjPerT];
+#pragma unroll
+        for (int k = 0; k < kHjPerT; k++) {
+            const uint32_t i = k * kJoinThreads + threadIdx.x;
+            if (i < tn0) rb[k] = part[v.index(0, i)];
+        }
+#pragma unroll
+        for (int k = 0; k < kHjPerT; k++) {
+            const uint32_t j = k * kJoinThreads + threadIdx.x;
+            if (j < pn0) rp[k] = part[v.index(1, j)];
+        }
+#pragma unroll
+        for (int k = 0; k < kHjPerT; k++) {
+            const uint32_t j = k * kJoinThreads + threadIdx.x;
+            if (j < pn0) s_probe[j] = rp[k];
+        }
+    }
+    uint32_t resident = 0;  // first element of the probe chunk in s_probe
+    for (uint32_t t0 = 0; t0 < nb; t0 += kHjCap) {
+        const uint32_t tn = min<uint32_t>(kHjCap, nb - t0);
+        if (t0 > 0) {
+            __syncthreads();
+            for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
 #pragma unroll
             for (int k = 0; k < kHjPerT; k++) {
                 const uint32_t i = k * kJoinThreads + threadIdx.x;
-                if (i < tn0) rb[k] = part[v.index(0, i)];
-            }
-#pragma unroll
-            for (int k = 0; k < kHjPerT; k++) {
-                const uint32_t j = k * kJoinThreads + threadIdx.x;
-                if (j < pn0) rp[k] = part[v.index(1, j)];
-            }
-#pragma unroll
-            for (int k = 0; k < kHjPerT; k++) {
-                const uint32_t j = k * kJoinThreads + threadIdx.x;
-                if (j < pn0) s_probe[j] = rp[k];
+                if (i < tn) rb[k] = part[v.index(0, t0 + i)];
             }
         }
-        uint32_t resident = 0;  // first element of the probe chunk in s_probe
-        for (uint32_t t0 = 0; t0 < nb; t0 += kHjCap) {
-            const uint32_t tn = min<uint32_t>(kHjCap, nb - t0);
-            if (t0 > 0) {
+        hj_build(t, rb, tn, bshift, s_ws);
+        for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
+            const uint32_t un = min<uint32_t>(kHjCap, np - u0);
+            if (u0 != resident) {
                 __syncthreads();
-                for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
-#pragma unroll
-                for (int k = 0; k < kHjPerT; k++) {
-                    const uint32_t i = k * kJoinThreads + threadIdx.x;
-                    if (i < tn) rb[k] = part[v.index(0, t0 + i)];
-                }
+                gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { s_probe[i] = r; });
+                resident = u0;
+                __syncthreads();
             }
-            hj_build(t, rb, tn, bshift, s_ws);
-            for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
-                const uint32_t un = min<uint32_t>(kHjCap, np - u0);
-                if (u0 != resident) {
-                    __syncthreads();
-                    gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { s_probe[i] = r; });
-                    resident = u0;
-                    __syncthreads();
-                }
-                hj_probe_lds<C2, WRITE>(t, s_probe, un, bshift, acc, L, R, out, out_cap, wrap, cursor,
-                                        s_ws, &s_base);
+            hj_probe_lds<C2, WRITE>(t, s_probe, un, bshift, acc, io.L, io.R, io.out, io.out_cap,
+                                    io.wrap, io.cursor, s_ws, s_base);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Aggregated (group-by-key) build: grouped mode, count + C1 only
+// ---------------------------------------------------------------------------
+// C1 factorises per key: sum over pairs of A(r)*B(s) = sum over probe
+// records s of B(s) * (sum of A over the build records with s's key).  The
+// build side of a partition is aggregated into an LDS open-addressing table
+// of distinct keys (key, count, sum of A); a probe record costs one lookup
+// whatever the key's multiplicity, so the work is O(n) at any skew.  The
+// table is filled in rounds of at most kJoinThreads records; before a round
+// starts with more than kAggHigh distinct keys in the table, every probe
+// record is looked up against it and it is cleared, so the table never
+// exceeds 3/4 load.  Key 0xFFFFFFFF (the empty marker) has its own slot.
+constexpr int kAggLog = 11;
+constexpr int kAggSlots = 1 << kAggLog;
+constexpr uint32_t kAggHigh = kAggSlots / 2;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+static_assert(kAggHigh + kJoinThreads <= (kAggSlots * 3) / 4, "aggregation table load");
+
+struct AggLds {
+    uint32_t key[kAggSlots];
+    uint32_t cnt[kAggSlots];
+    uint64_t suma[kAggSlots];
+    uint64_t ff_suma;  // key 0xFFFFFFFF
+    uint32_t ff_cnt;
+    uint32_t distinct;
+    Rec probe[kHjCap];
+};
+
+__device__ __forceinline__ uint32_t agg_slot(uint32_t k, int bshift) {
+    return (key_hash(k) << bshift) >> (32 - kAggLog);
+}
+
+// Every thread of the block; the caller puts a barrier after it.
+__device__ __forceinline__ void agg_clear(AggLds &t) {
+    for (int i = threadIdx.x; i < kAggSlots; i += kJoinThreads) {
+        t.key[i] = kEmptyKey;
+        t.cnt[i] = 0;
+        t.suma[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+        t.ff_suma = 0;
+        t.ff_cnt = 0;
+        t.distinct = 0;
+    }
+}
+
+__device__ __forceinline__ void agg_add(AggLds &t, uint32_t k, uint32_t cnt, uint64_t sa, int bshift) {
+    if (k == kEmptyKey) {
+        atomicAdd(&t.ff_cnt, cnt);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&t.ff_suma), (unsigned long long)sa);
+        return;
+    }
+    uint32_t s = agg_slot(k, bshift);
+    for (;;) {
+        uint32_t cur = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == kEmptyKey) {
+            cur = atomicCAS(&t.key[s], kEmptyKey, k);
+            if (cur == kEmptyKey) {
+                atomicAdd(&t.distinct, 1u);
+                cur = k;
             }
-            __syncthreads();
+        }
+        if (cur == k) break;
+        s = (s + 1) & (kAggSlots - 1);
+    }
+    atomicAdd(&t.cnt[s], cnt);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&t.suma[s]), (unsigned long long)sa);
+}
+
+// Inserts one record per lane (`valid` lanes).  The lanes sharing the first
+// valid lane's key are combined into one table update (the common case
+// under heavy skew, where one key dominates a partition).
+__device__ __forceinline__ void agg_insert(AggLds &t, bool valid, Rec r, int bshift) {
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t a = valid ? hash_a(r.x, r.y) : 0ull;
+    const int lead = __ffsll((unsigned long long)vmask) - 1;
+    const uint32_t k0 = __shfl(r.x, lead, kWave);
+    const bool same = valid && r.x == k0;
+    const uint64_t smask = __ballot(same);
+    const uint64_t sa = wave_sum(same ? a : 0ull);
+    if (lane == lead) agg_add(t, k0, uint32_t(__popcll(smask)), sa, bshift);
+    if (valid && !same) agg_add(t, r.x, 1u, a, bshift);
+}
+
+__device__ __forceinline__ void agg_probe(const AggLds &t, Rec s, int bshift, Acc &acc) {
+    uint32_t cnt = 0;
+    uint64_t sa = 0;
+    if (s.x == kEmptyKey) {
+        cnt = t.ff_cnt;
+        sa = t.ff_suma;
+    } else {
+        uint32_t sl = agg_slot(s.x, bshift);
+        for (;;) {
+            const uint32_t cur = t.key[sl];
+            if (cur == s.x) {
+                cnt = t.cnt[sl];
+                sa = t.suma[sl];
+                break;
+            }
+            if (cur == kEmptyKey) break;
+            sl = (sl + 1) & (kAggSlots - 1);
         }
     }
-    // one partial per wave (summed by k_reduce): no block-level reduction
-    acc.count = wave_sum(acc.count);
-    acc.c1 = wave_sum(acc.c1);
-    acc.c2 = wave_sum(acc.c2);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const uint64_t w = uint64_t(p) * (kJoinThreads / kWave) + threadIdx.x / kWave;
-        partials[3 * w] = acc.count;
-        partials[3 * w + 1] = acc.c1;
-        partials[3 * w + 2] = acc.c2;
+    if (cnt) {
+        acc.count += cnt;
+        acc.c1 += sa * hash_b(s.x, s.y);
     }
+}
+
+__device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &v, int bshift,
+                                              AggLds &t, Acc &acc) {
+    const uint32_t nb = v.n[0], np = v.n[1];
+    if (!nb || !np) return;
+    const Rec *__restrict__ part = io.part;
+    const uint32_t pn0 = min<uint32_t>(np, kHjCap);
+    agg_clear(t);
+    gather(part, v, 1, 0, pn0, [&](uint32_t i, Rec r) { t.probe[i] = r; });
+    __syncthreads();
+    uint32_t resident = 0;
+    // probe every probe record against the table
+    auto probe_all = [&]() {
+        for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
+            const uint32_t un = min<uint32_t>(kHjCap, np - u0);
+            if (u0 != resident) {
+                __syncthreads();
+                gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { t.probe[i] = r; });
+                resident = u0;
+                __syncthreads();
+            }
+            for (uint32_t j = threadIdx.x; j < un; j += kJoinThreads) agg_probe(t, t.probe[j], bshift, acc);
+        }
+    };
+    for (uint32_t i0 = 0; i0 < nb; i0 += kJoinThreads * kBatch) {
+        Rec r[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const uint32_t i = i0 + k * kJoinThreads + threadIdx.x;
+            r[k] = i < nb ? part[v.index(0, i)] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            if (i0 + k * kJoinThreads >= nb) break;  // uniform
+            __syncthreads();
+            const bool full = t.distinct > kAggHigh;
+            __syncthreads();
+            if (full) {
+                probe_all();
+                __syncthreads();
+                agg_clear(t);
+                __syncthreads();
+            }
+            const uint32_t i = i0 + k * kJoinThreads + threadIdx.x;
+            agg_insert(t, i < nb, r[k], bshift);
+        }
+    }
+    __syncthreads();
+    probe_all();
 }
 
 // ---------------------------------------------------------------------------
@@ -444,14 +606,18 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *s, int n) {
     }
 }
 
-// Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload),
-// pads to a power of two with ~0 and sorts.  Padding entries sort last and
-// equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit for bit, so the first n
-// sorted entries are exactly the tile.
-__device__ __forceinline__ void gather_sorted(uint64_t *s, const Rec *__restrict__ part,
-                                              const PartView &v, int side, uint32_t t0, int n) {
-    int np2 = 2;
-    while (np2 < n) np2 <<= 1;
+// Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload).
+// SORT: pads to a power of two with ~0 and sorts; padding entries sort last
+// and equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit for bit, so the first
+// n sorted entries are exactly the tile.  Ends with a barrier.
+template <bool SORT>
+__device__ __forceinline__ void gather_tile(uint64_t *s, const Rec *__restrict__ part,
+                                            const PartView &v, int side, uint32_t t0, int n) {
+    int np2 = n;
+    if (SORT) {
+        np2 = 2;
+        while (np2 < n) np2 <<= 1;
+    }
     for (int i0 = 0; i0 < np2; i0 += kJoinThreads * kBatch) {
         Rec r[kBatch];
 #pragma unroll
@@ -466,7 +632,7 @@ __device__ __forceinline__ void gather_sorted(uint64_t *s, const Rec *__restrict
         }
     }
     __syncthreads();
-    bitonic_sort(s, np2);
+    if (SORT) bitonic_sort(s, np2);
 }
 
 __device__ __forceinline__ int lower_bound_key(const uint64_t *s, int n, uint32_t k) {
@@ -493,30 +659,27 @@ __device__ __forceinline__ int upper_bound_key(const uint64_t *s, int lo, int n,
     return lo;
 }
 
-// GROUPED: count + C1 via run closed forms (C2 not produced).
+struct SmjLds {
+    uint64_t b[kSmjCap];      // sorted build tile
+    uint64_t p[kSmjCap];      // probe tile (unsorted)
+    uint64_t pa[kSmjCap + 1]; // grouped: prefix sums of A over the sorted build tile
+};
+
+// Joins one partition (SMJ): each build tile is bitonic-sorted in LDS and
+// every probe element is merged into it by co-ranking its key (lower /
+// upper bound: the one-sided merge path), so the probe tile itself needs
+// no sort.  GROUPED: count + C1 from prefix sums of A over the sorted tile;
+// a grouped partition whose build side exceeds one tile (heavy keys) goes
+// to the aggregated build instead, which is O(n) at any skew.
 template <bool C2, bool WRITE, bool GROUPED>
-__global__ __launch_bounds__(kJoinThreads) void k_smj_join(
-    const Rec *__restrict__ part, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR,
-    uint64_t nL, const sjb_tuple_t *__restrict__ L, const sjb_tuple_t *__restrict__ R,
-    uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out, uint64_t out_cap,
-    int wrap, unsigned long long *__restrict__ cursor) {
-    __shared__ uint64_t s_b[kSmjCap];
-    __shared__ uint64_t s_p[kSmjCap];
-    __shared__ uint64_t s_pa[GROUPED ? kSmjCap + 1 : 1];
-    extern __shared__ uint32_t dyn[];
-    __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
-    __shared__ uint32_t s_ws[kJoinThreads / kWave];
-    __shared__ unsigned long long s_base;
-
-    const int p = xcd_partition(blockIdx.x, 1 << logF);
-    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, nL, kSmjCap, s_ws);
+__device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &v, SmjLds &m,
+                                              uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
-    Acc acc;
-    acc.zero();
-
+    const Rec *__restrict__ part = io.part;
+    uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_sorted(s_b, part, v, 0, t0, tn);
+        gather_tile<true>(s_b, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -535,9 +698,9 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
             if (threadIdx.x == 0) {
                 uint64_t run = 0;
                 for (int t = 0; t < kJoinThreads; t++) {
-                    const uint64_t v = s_pa[t];
+                    const uint64_t x = s_pa[t];
                     s_pa[t] = run;
-                    run += v;
+                    run += x;
                 }
             }
             __syncthreads();
@@ -553,7 +716,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
         }
         for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
             const int un = int(min<uint32_t>(kSmjCap, np - u0));
-            gather_sorted(s_p, part, v, 1, u0, un);
+            gather_tile<false>(s_p, part, v, 1, u0, un);
             if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
@@ -586,18 +749,18 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
                     uint32_t tot;
                     const uint32_t ex = block_excl_scan_u32(uint32_t(hi - lo), s_ws, &tot);
                     if (threadIdx.x == 0)
-                        s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+                        *s_base = tot ? atomicAdd(io.cursor, (unsigned long long)tot) : 0ull;
                     __syncthreads();
                     if (hi > lo) {
-                        uint64_t pos = s_base + ex;
-                        const sjb_tuple_t st = R[si];
+                        uint64_t pos = *s_base + ex;
+                        const sjb_tuple_t st = io.R[si];
                         for (int e = lo; e < hi; e++) {
-                            const sjb_tuple_t lt = L[uint32_t(s_b[e])];
+                            const sjb_tuple_t lt = io.L[uint32_t(s_b[e])];
                             const uint64_t a = hash_a(lt.key, lt.val);
                             acc.count++;
                             acc.c1 += a * hash_b(st.key, st.val);
                             if (C2) acc.c2 += pair_hash_from_a(a, st.val);
-                            write_row(out, pos++, out_cap, wrap, lt, st);
+                            write_row(io.out, pos++, io.out_cap, io.wrap, lt, st);
                         }
                     }
                     __syncthreads();
@@ -606,11 +769,132 @@ __global__ __launch_bounds__(kJoinThreads) void k_smj_join(
             __syncthreads();
         }
     }
-    const Acc r = block_sum(acc, s_red);
-    if (threadIdx.x == 0) {
-        partials[3 * p] = r.count;
-        partials[3 * p + 1] = r.c1;
-        partials[3 * p + 2] = r.c2;
+}
+
+// LDS bytes of each join body.
+template <int ALG, bool GROUPED>
+struct JoinLdsBytes {
+    static constexpr size_t hj = GROUPED ? sizeof(AggLds) : sizeof(HjLds);
+    static constexpr size_t smj = GROUPED ? (sizeof(SmjLds) > sizeof(AggLds) ? sizeof(SmjLds) : sizeof(AggLds))
+                                          : sizeof(SmjLds);
+    static constexpr size_t value = ALG == SJB_ALG_SMJ ? smj : hj;
+};
+
+// Joins partition view v with the body of (ALG, C2, WRITE, GROUPED).
+template <int ALG, bool C2, bool WRITE, bool GROUPED>
+__device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, int bshift, uint8_t *mem,
+                                          uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
+    if constexpr (ALG == SJB_ALG_HJ) {
+        if constexpr (GROUPED)
+            agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
+        else
+            hj_join_part<C2, WRITE>(io, v, bshift, *reinterpret_cast<HjLds *>(mem), s_ws, s_base, acc);
+    } else {
+        if (GROUPED && v.n[0] > uint32_t(kSmjCap))
+            agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
+        else
+            smj_join_part<C2, WRITE, GROUPED>(io, v, *reinterpret_cast<SmjLds *>(mem), s_ws, s_base, acc);
+    }
+}
+
+template <int ALG>
+__host__ __device__ constexpr int idcap_of() {
+    return ALG == SJB_ALG_SMJ ? kSmjCap : kHjCap;
+}
+
+// ---------------------------------------------------------------------------
+// one window: one workgroup per partition
+// ---------------------------------------------------------------------------
+// Per-wave partials go to partials[(p * kPartialsPerPart + wave) * 3 ...]
+// and are summed by k_reduce.
+template <int ALG, bool C2, bool WRITE, bool GROUPED>
+__global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t *__restrict__ offs,
+                                                       int logF, int ncL, int ncR, uint32_t nL,
+                                                       int bshift, uint64_t *__restrict__ partials) {
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED>::value];
+    __shared__ uint32_t s_ws[kJoinThreads / kWave];
+    __shared__ unsigned long long s_base;
+    extern __shared__ uint32_t dyn[];
+
+    const int p = xcd_partition(blockIdx.x, 1 << logF);
+    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG>(), s_ws);
+    Acc acc;
+    acc.zero();
+    join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
+    // one partial per wave (summed by k_reduce): no block-level reduction
+    acc.count = wave_sum(acc.count);
+    acc.c1 = wave_sum(acc.c1);
+    acc.c2 = wave_sum(acc.c2);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const uint64_t w = uint64_t(p) * kPartialsPerPart + threadIdx.x / kWave;
+        partials[3 * w] = acc.count;
+        partials[3 * w + 1] = acc.c1;
+        partials[3 * w + 2] = acc.c2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched windows: one persistent launch over (window, partition) items
+// ---------------------------------------------------------------------------
+// The host deals the windows over kXcds queues (largest first, each to the
+// least loaded queue) and orders every queue by decreasing window size.
+// Workgroup b starts on queue b % kXcds — the XCD the hardware dispatched it
+// to, so neighbouring partitions of a window (which share cache lines of
+// every chunk's runs) are joined under one L2 — takes `grab` items per
+// atomic step and, once its queue is empty, drains the others in turn.
+// Every workgroup visits every queue once and exits when all are empty, so
+// the grid always drains.  Each item adds its (count, C1, C2) to its
+// window's result with three 64-bit atomics (sums mod 2^64: any order gives
+// the same bits).
+template <int ALG, bool C2, bool GROUPED>
+__global__ __launch_bounds__(kJoinThreads) void k_join_batch(BatchArgs a) {
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED>::value];
+    __shared__ uint32_t s_ws[kJoinThreads / kWave];
+    __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_item;
+    extern __shared__ uint32_t dyn[];
+
+    JoinIO io;
+    io.part = a.part;
+    io.L = a.L;
+    io.R = a.R;
+    io.out = nullptr;
+    io.out_cap = 0;
+    io.wrap = 0;
+    io.cursor = nullptr;
+    const int home = blockIdx.x % kXcds;
+    for (int qi = 0; qi < kXcds; qi++) {
+        const int x = (home + qi) % kXcds;
+        const uint32_t qlen = a.qbeg[x + 1] - a.qbeg[x];
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_item = qlen ? atomicAdd(&a.qhead[x * 16], a.grab) : qlen;
+            __syncthreads();
+            const uint32_t i0 = s_item;
+            if (i0 >= qlen) break;
+            const uint32_t i1 = min(qlen, i0 + a.grab);
+            for (uint32_t i = i0; i < i1; i++) {
+                const uint32_t item = a.qbeg[x] + i;
+                const BatchWin bw = a.win[a.item_win[item]];
+                const int p = int(item - bw.item_base);
+                const int logF = int(bw.logF);
+                __syncthreads();  // the previous item's LDS is no longer read
+                const PartView v = load_view(dyn, a.offs + bw.offs_base, logF, p, int(bw.ncL),
+                                             int(bw.ncR), bw.lpart, bw.rpart, a.chunk, a.idcap, s_ws);
+                Acc acc;
+                acc.zero();
+                const int bshift = ALG == SJB_ALG_HJ ? logF : 0;
+                join_part<ALG, C2, false, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
+                const Acc r = block_sum(acc, s_red);
+                if (threadIdx.x == 0 && (r.count | r.c1 | r.c2)) {
+                    sjb_window_result_t *o = a.res + bw.id;
+                    atomicAdd(reinterpret_cast<unsigned long long *>(&o->count), (unsigned long long)r.count);
+                    atomicAdd(reinterpret_cast<unsigned long long *>(&o->checksum1), (unsigned long long)r.c1);
+                    atomicAdd(reinterpret_cast<unsigned long long *>(&o->checksum2), (unsigned long long)r.c2);
+                }
+            }
+        }
     }
 }
 
@@ -657,58 +941,80 @@ int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out) {
     return SJB_OK;
 }
 
-int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
-                   sjb_result_tuple_t *out, uint64_t out_cap, int wrap) {
+int join_idcap(int alg) { return alg == SJB_ALG_SMJ ? kSmjCap : kHjCap; }
+
+size_t join_view_bytes(int alg, int ncL, int ncR) { return part_view_bytes(ncL, ncR, join_idcap(alg)); }
+
+template <int ALG>
+static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
+                           sjb_result_tuple_t *out, uint64_t out_cap, int wrap) {
     const int F = 1 << p.logF;
     const bool c2 = (p.flags & SJB_F_C2) != 0;
+    const bool grouped = (p.flags & SJB_F_GROUPED) != 0 && p.payload == PAY_VAL;
     const int bshift = p.kind == PART_HASH ? p.logF : 0;
-    const size_t dynb = part_view_bytes(p.ncL, p.ncR, kHjCap);
-    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
-#define SJB_HJ(C2_, W_)                                                                           \
-    k_hj_join<C2_, W_><<<F, kJoinThreads, dynb, c->stream>>>(c->d_part, c->d_offs, p.logF, p.ncL,     \
-                                                          p.ncR, p.nL, bshift, L, R, c->d_partials, \
-                                                          out, out_cap, wrap, cur)
+    const size_t dynb = part_view_bytes(p.ncL, p.ncR, idcap_of<ALG>());
+    JoinIO io;
+    io.part = c->d_part;
+    io.L = L;
+    io.R = R;
+    io.out = out;
+    io.out_cap = out_cap;
+    io.wrap = wrap;
+    io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
+#define SJB_J(C2_, W_, G_)                                                                          \
+    k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, c->d_offs, p.logF, p.ncL,   \
+                                                                 p.ncR, uint32_t(p.nL), bshift,    \
+                                                                 c->d_partials)
     if (p.payload == PAY_IDX) {
         if (c2)
-            SJB_HJ(true, true);
+            SJB_J(true, true, false);
         else
-            SJB_HJ(false, true);
+            SJB_J(false, true, false);
+    } else if (grouped) {
+        SJB_J(false, false, true);
+    } else if (c2) {
+        SJB_J(true, false, false);
     } else {
-        if (c2)
-            SJB_HJ(true, false);
-        else
-            SJB_HJ(false, false);
+        SJB_J(false, false, false);
     }
-#undef SJB_HJ
+#undef SJB_J
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
 
+int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
+                   sjb_result_tuple_t *out, uint64_t out_cap, int wrap) {
+    return launch_join_alg<SJB_ALG_HJ>(c, p, L, R, out, out_cap, wrap);
+}
+
 int launch_smj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
                     sjb_result_tuple_t *out, uint64_t out_cap, int wrap) {
-    const int F = 1 << p.logF;
-    const bool c2 = (p.flags & SJB_F_C2) != 0;
-    const bool grouped = (p.flags & SJB_F_GROUPED) != 0 && p.payload == PAY_VAL;
-    const size_t dynb = part_view_bytes(p.ncL, p.ncR, kSmjCap);
-    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
-#define SJB_SMJ(C2_, W_, G_)                                                                     \
-    k_smj_join<C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(c->d_part, c->d_offs, p.logF,     \
-                                                               p.ncL, p.ncR, p.nL, L, R,         \
-                                                               c->d_partials, out, out_cap, wrap, \
-                                                               cur)
-    if (p.payload == PAY_IDX) {
-        if (c2)
-            SJB_SMJ(true, true, false);
+    return launch_join_alg<SJB_ALG_SMJ>(c, p, L, R, out, out_cap, wrap);
+}
+
+int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a, int blocks,
+                      size_t dyn) {
+    const bool c2 = (flags & SJB_F_C2) != 0;
+    const bool grouped = (flags & SJB_F_GROUPED) != 0;
+#define SJB_JB(A_, C2_, G_) k_join_batch<A_, C2_, G_><<<blocks, kJoinThreads, dyn, s>>>(a)
+    if (alg == SJB_ALG_HJ) {
+        if (grouped)
+            SJB_JB(SJB_ALG_HJ, false, true);
+        else if (c2)
+            SJB_JB(SJB_ALG_HJ, true, false);
         else
-            SJB_SMJ(false, true, false);
-    } else if (grouped) {
-        SJB_SMJ(false, false, true);
-    } else if (c2) {
-        SJB_SMJ(true, false, false);
+            SJB_JB(SJB_ALG_HJ, false, false);
+    } else if (alg == SJB_ALG_SMJ) {
+        if (grouped)
+            SJB_JB(SJB_ALG_SMJ, false, true);
+        else if (c2)
+            SJB_JB(SJB_ALG_SMJ, true, false);
+        else
+            SJB_JB(SJB_ALG_SMJ, false, false);
     } else {
-        SJB_SMJ(false, false, false);
+        return SJB_E_UNSUPPORTED;
     }
-#undef SJB_SMJ
+#undef SJB_JB
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }

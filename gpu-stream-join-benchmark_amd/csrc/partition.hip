@@ -26,22 +26,27 @@ template <int KIND>
 struct PartFn {
     uint32_t logF, fmask, kmin, shift;
     const RmiParams *rmi;
-    __device__ __forceinline__ void init(int logF_, const PartParams *pp, const RmiParams *rmi_) {
+    // kmin / shift: range partitioning parameters (ignored by HASH/LEARNED)
+    __device__ __forceinline__ void init(int logF_, uint32_t kmin_, uint32_t shift_,
+                                         const RmiParams *rmi_) {
         logF = logF_;
         fmask = (1u << logF_) - 1u;
         rmi = rmi_;
         if (KIND == PART_RANGE) {
-            kmin = pp->kmin;
-            shift = pp->shift;
+            kmin = kmin_;
+            shift = shift_;
         } else {
             kmin = 0;
             shift = 32 - logF_;
         }
     }
     __device__ __forceinline__ uint32_t operator()(uint32_t k) const {
-        if (KIND == PART_HASH) return key_hash(k) >> shift;
+        if (KIND == PART_HASH) return logF ? key_hash(k) >> shift : 0u;
         if (KIND == PART_RANGE) {
-            uint32_t p = (k - kmin) >> shift;
+            // keys outside [kmin, ...] clamp to the end partitions: the
+            // function stays a function of the key, so equal keys meet
+            if (k < kmin) return 0u;
+            const uint32_t p = (k - kmin) >> shift;
             return p > fmask ? fmask : p;
         }
         // LEARNED
@@ -125,38 +130,27 @@ int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple
 // ---------------------------------------------------------------------------
 // single-pass chunk-major partitioning
 // ---------------------------------------------------------------------------
-// One workgroup of kPartThreads threads owns one chunk of kChunk tuples of
-// one side.  It loads its tuples (16 per thread, coalesced), ranks them in
-// an LDS histogram over F partitions, scans the histogram, places the 8-byte
-// records partition-sorted into an LDS staging buffer and writes the whole
-// chunk back contiguously — every global write is a full, coalesced line.
-// The chunk's local partition offsets (F+1 entries) go to
-// offs[chunk_id * (F+1) + p]; chunk_id runs over left chunks then right
-// chunks.  The join gathers partition p as the run
-// [offs[c][p], offs[c][p+1]) of every chunk c.
-template <int KIND, int PAY>
-__global__ __launch_bounds__(kPartThreads) void k_partition(
-    const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
-    extern __shared__ uint32_t lds[];
+// One workgroup of kPartThreads threads owns one chunk of up to CHUNK
+// tuples of one side.  It loads its tuples (CHUNK/kPartThreads per thread,
+// coalesced), ranks them in an LDS histogram over F partitions, scans the
+// histogram, places the 8-byte records partition-sorted into an LDS staging
+// buffer and writes the whole chunk back contiguously — every global write
+// is a full, coalesced line.  The chunk's local partition offsets (F+1
+// entries) go to `o`.  The join gathers partition p as the run
+// [o_c[p], o_c[p+1]) of every chunk c.
+template <int KIND, int PAY, int CHUNK>
+__device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ in, uint64_t base,
+                                                int len, int logF, const PartFn<KIND> &part,
+                                                uint32_t *__restrict__ o, Rec *__restrict__ out,
+                                                uint32_t *lds) {
     const int F = 1 << logF;
     uint32_t *hist = lds;                                  // F + 1 counters
     uint32_t *ws = lds + F + 1;                            // wave sums
     Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 1 + kPartThreads / kWave + 1) & ~1));
-    PartFn<KIND> part;
-    part.init(logF, pp, rmi);
-    const bool left = int(blockIdx.x) < ncL;
-    const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
-    const sjb_tuple_t *in = left ? L : R;
-    const uint64_t n = left ? nL : nR;
-    const uint64_t base = uint64_t(chunk) * kChunk;
-    const int len = int(min<uint64_t>(kChunk, n - base));
-    Rec *out = part_out + (left ? 0 : nL) + base;
 
     for (int q = threadIdx.x; q <= F; q += kPartThreads) hist[q] = 0;
     __syncthreads();
-    constexpr int kPer = kChunk / kPartThreads;
+    constexpr int kPer = CHUNK / kPartThreads;
     Rec rec[kPer];
     uint32_t pid[kPer], rank[kPer];
 #pragma unroll
@@ -197,7 +191,6 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
         if (threadIdx.x == 0) hist[F] = tot;
     }
     __syncthreads();
-    uint32_t *o = offs + uint64_t(blockIdx.x) * (F + 1);
     for (int q = threadIdx.x; q <= F; q += kPartThreads) o[q] = hist[q];
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
@@ -208,20 +201,138 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
     for (int e = threadIdx.x; e < len; e += kPartThreads) out[e] = stage[e];
 }
 
-int choose_logF(int alg, uint64_t n_build, int requested) {
-    if (requested > 0) return std::min(std::max(requested, 1), kMaxLogF);
-    // expected partition size at most ~0.92 (HJ) / 0.5 (SMJ) of an LDS tile,
-    // so uniform keys never spill into a second tile
-    const uint64_t target = alg == SJB_ALG_SMJ ? kSmjCap / 2 : (kHjCap * 23) / 25;
-    int logF = 1;
-    while (logF < kMaxLogF && (n_build >> logF) > target) logF++;
-    return logF;
+// One window: blocks [0, ncL) take left chunks and [ncL, ncL+ncR) right
+// chunks; chunk b's offsets row is offs[b * (F+1)].
+template <int KIND, int PAY>
+__global__ __launch_bounds__(kPartThreads) void k_partition(
+    const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
+    uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    extern __shared__ uint32_t lds[];
+    PartFn<KIND> part;
+    part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
+    const bool left = int(blockIdx.x) < ncL;
+    const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
+    const uint64_t n = left ? nL : nR;
+    const uint64_t base = uint64_t(chunk) * kChunk;
+    const int len = int(min<uint64_t>(kChunk, n - base));
+    partition_chunk<KIND, PAY, kChunk>(left ? L : R, base, len, logF, part,
+                                       offs + uint64_t(blockIdx.x) * ((1 << logF) + 1),
+                                       part_out + (left ? 0 : nL) + base, lds);
 }
 
-size_t partition_lds_bytes(int logF) {
-    const size_t words = ((size_t(1) << logF) + 1 + kPartThreads / kWave + 1) & ~size_t(1);
-    return words * 4 + size_t(kChunk) * sizeof(Rec);
+// Range parameters of a batched window from a fixed strided sample of up
+// to 256 keys per side.  Every chunk workgroup of the window draws the same
+// sample, so all of them use the same partition function; keys outside the
+// sampled range clamp to the end partitions (exact, only the balance
+// depends on the sample).
+__device__ __forceinline__ void sample_range(const sjb_tuple_t *__restrict__ L, uint32_t nL,
+                                             const sjb_tuple_t *__restrict__ R, uint32_t nR,
+                                             int logF, uint32_t *s_mm, uint32_t *kmin,
+                                             uint32_t *shift) {
+    constexpr int kS = 256;
+    if (threadIdx.x < 2) s_mm[threadIdx.x] = threadIdx.x ? 0u : 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    const int t = threadIdx.x;
+    if (t < 2 * kS) {
+        const bool left = t < kS;
+        const uint32_t n = left ? nL : nR;
+        const uint32_t j = left ? t : t - kS;
+        const uint32_t ns = n < uint32_t(kS) ? n : uint32_t(kS);
+        if (j < ns) {
+            const uint32_t idx = uint32_t((uint64_t(j) * n) / ns);
+            const uint32_t k = (left ? L : R)[idx].key;
+            mn = k;
+            mx = k;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, kWave));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && t < 2 * kS) {
+        atomicMin(&s_mm[0], mn);
+        atomicMax(&s_mm[1], mx);
+    }
+    __syncthreads();
+    mn = s_mm[0];
+    mx = s_mm[1];
+    if (mn > mx) mn = mx = 0;
+    const uint32_t span = mx - mn;
+    const int bits = span ? 32 - __clz(span) : 0;
+    *kmin = mn;
+    *shift = bits > logF ? uint32_t(bits - logF) : 0u;
 }
+
+// Batched windows: block b is partition-pass item b of the batch (one chunk
+// of one side of one window).
+template <int KIND, int CHUNK>
+__global__ __launch_bounds__(kPartThreads) void k_partition_batch(BatchArgs a) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_mm[2];
+    const uint32_t item = blockIdx.x;
+    const BatchWin bw = a.win[a.chunk_win[item]];
+    const uint32_t local = item - bw.chunk_base;
+    const bool left = local < bw.ncL;
+    const uint32_t chunk = left ? local : local - bw.ncL;
+    const int logF = int(bw.logF);
+    uint32_t kmin = 0, shift = 0;
+    if (KIND == PART_RANGE)
+        sample_range(a.L + bw.lo, bw.nL, a.R + bw.ro, bw.nR, logF, s_mm, &kmin, &shift);
+    PartFn<KIND> part;
+    part.init(logF, kmin, shift, a.rmi);
+    const uint32_t n = left ? bw.nL : bw.nR;
+    const uint64_t base = uint64_t(chunk) * CHUNK;
+    const int len = int(min<uint64_t>(CHUNK, n - base));
+    partition_chunk<KIND, PAY_VAL, CHUNK>(
+        left ? a.L + bw.lo : a.R + bw.ro, base, len, logF, part,
+        a.offs + bw.offs_base + uint64_t(local) * ((1u << logF) + 1),
+        a.part + (left ? bw.lpart : bw.rpart) + base, lds);
+}
+
+// Zeroes the per-window results and the work-queue heads of a batch (a
+// kernel, so a captured graph holds no memset node).
+__global__ void k_batch_clear(sjb_window_result_t *__restrict__ res, uint32_t n,
+                              uint32_t *__restrict__ qhead) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        res[i].count = 0;
+        res[i].checksum1 = 0;
+        res[i].checksum2 = 0;
+        res[i].pairs = 0;
+    }
+    if (i < kXcds * 16) qhead[i] = 0;
+}
+
+// log2 fanout for a window whose build side has n_build tuples: the
+// expected partition size stays below what one LDS tile holds (HJ: ~0.92 of
+// the CSR table, SMJ: half a sort tile, grouped HJ: 3/8 of the aggregation
+// table), so uniform keys never spill into a second tile.  `bias` shifts
+// the choice (autotuner); the result is clamped to [0, kMaxLogF].
+int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias) {
+    uint64_t target = (kHjCap * 23) / 25;
+    if (alg == SJB_ALG_SMJ)
+        target = kSmjCap / 2;
+    else if (flags & SJB_F_GROUPED)
+        target = 768;
+    int logF = 0;
+    while (logF < kMaxLogF && (n_build >> logF) > target) logF++;
+    logF += bias;
+    return std::min(std::max(logF, 0), kMaxLogF);
+}
+
+int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested) {
+    if (requested > 0) return std::min(std::max(requested, 1), kMaxLogF);
+    return std::max(batch_log_fanout(alg, flags, n_build, 0), 1);
+}
+
+size_t partition_lds_bytes_chunk(int logF, int chunk) {
+    const size_t words = ((size_t(1) << logF) + 1 + kPartThreads / kWave + 1) & ~size_t(1);
+    return words * 4 + size_t(chunk) * sizeof(Rec);
+}
+size_t partition_lds_bytes(int logF) { return partition_lds_bytes_chunk(logF, kChunk); }
 
 template <int KIND>
 static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
@@ -249,23 +360,57 @@ int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, cons
     return SJB_E_INVALID;
 }
 
+template <int KIND, int CHUNK>
+static int partition_batch_kind(hipStream_t s, const BatchArgs &a, uint32_t n_chunks, size_t dyn) {
+    k_partition_batch<KIND, CHUNK><<<n_chunks, kPartThreads, dyn, s>>>(a);
+    SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
+int launch_partition_batch(hipStream_t s, int kind, const BatchArgs &a, uint32_t n_chunks,
+                           size_t dyn) {
+    if (n_chunks == 0) return SJB_OK;
+    const bool big = a.chunk == kChunk;
+    if (!big && a.chunk != 4096) return SJB_E_INVALID;
+    switch (kind) {
+        case PART_HASH:
+            return big ? partition_batch_kind<PART_HASH, kChunk>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_HASH, 4096>(s, a, n_chunks, dyn);
+        case PART_RANGE:
+            return big ? partition_batch_kind<PART_RANGE, kChunk>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_RANGE, 4096>(s, a, n_chunks, dyn);
+        case PART_LEARNED:
+            return big ? partition_batch_kind<PART_LEARNED, kChunk>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_LEARNED, 4096>(s, a, n_chunks, dyn);
+    }
+    return SJB_E_INVALID;
+}
+
+int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead) {
+    const uint32_t m = std::max<uint32_t>(n, kXcds * 16);
+    k_batch_clear<<<(m + 255) / 256, 256, 0, s>>>(res, n, qhead);
+    SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
 // Raises the dynamic-LDS limit of the partition kernels (the staging buffer
-// is 128 KiB); called once per context.
+// is up to 128 KiB); called once per context.
 int partition_setup(sjb_ctx *c) {
     const int need = int(partition_lds_bytes(kMaxLogF));
     (void)c;
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_HASH, PAY_VAL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_HASH, PAY_IDX>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_RANGE, PAY_VAL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_RANGE, PAY_IDX>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_LEARNED, PAY_VAL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
-    (void)hipFuncSetAttribute((const void *)k_partition<PART_LEARNED, PAY_IDX>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, need);
+    const void *fns[] = {
+        (const void *)k_partition<PART_HASH, PAY_VAL>,    (const void *)k_partition<PART_HASH, PAY_IDX>,
+        (const void *)k_partition<PART_RANGE, PAY_VAL>,   (const void *)k_partition<PART_RANGE, PAY_IDX>,
+        (const void *)k_partition<PART_LEARNED, PAY_VAL>, (const void *)k_partition<PART_LEARNED, PAY_IDX>,
+        (const void *)k_partition_batch<PART_HASH, kChunk>,
+        (const void *)k_partition_batch<PART_RANGE, kChunk>,
+        (const void *)k_partition_batch<PART_LEARNED, kChunk>,
+        (const void *)k_partition_batch<PART_HASH, 4096>,
+        (const void *)k_partition_batch<PART_RANGE, 4096>,
+        (const void *)k_partition_batch<PART_LEARNED, 4096>,
+    };
+    for (const void *f : fns)
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, need);
     (void)hipGetLastError();
     return SJB_OK;
 }

@@ -74,6 +74,47 @@ struct PartParams {
     uint32_t heavy[kMaxHeavy];
 };
 
+// One window of a batched launch (device descriptor, built on the host).
+// Records of the window's left side are partitioned into
+// d_part[lpart, lpart + nL), right side into d_part[rpart, rpart + nR).
+struct BatchWin {
+    uint64_t lo, ro;         // first tuple of the window in the left / right stream
+    uint64_t offs_base;      // first word of the window's chunk-offsets table
+    uint32_t lpart, rpart;   // first partitioned record of each side
+    uint32_t nL, nR;
+    uint32_t ncL, ncR;       // partition-pass chunks per side
+    uint32_t chunk_base;     // first partition-pass item of the window
+    uint32_t item_base;      // first join item (= partition) of the window
+    uint32_t id;             // caller's window index (result slot)
+    uint32_t logF;
+};
+
+// Arguments of the batched partition and join kernels (by value).
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin
+struct BatchArgs {
+    const sjb_tuple_t *L, *R;
+    const BatchWin *win;
+    const uint32_t *chunk_win;   // partition-pass item -> window
+    const uint32_t *item_win;    // join item -> window
+    Rec *part;
+    uint32_t *offs;
+    sjb_window_result_t *res;    // indexed by BatchWin::id
+    uint32_t *qhead;             // per-XCD queue heads, 64 B apart
+    uint32_t qbeg[kXcds + 1];    // join items of XCD x: [qbeg[x], qbeg[x+1])
+    uint32_t chunk;              // partition-pass chunk (tuples)
+    uint32_t grab;               // join items taken per queue access
+    uint32_t idcap;              // run-id table entries of a PartView
+    const RmiParams *rmi;
+};
+
+// Launch configuration of the batched path, chosen by sjb_batch_autotune.
+struct BatchTune {
+    int fanout_bias = 0;       // added to the per-window log2 fanout
+    int chunk_log2 = 14;       // partition-pass chunk: 2^12 or 2^14 tuples
+    int blocks_per_cu = 2;     // resident join workgroups per CU
+    int grab = 1;              // join items per queue access
+};
+
 struct WindowPlan {
     int alg;
     int logF;
@@ -119,6 +160,8 @@ struct sjb_ctx {
     // batched windows
     sjb_window_result_t *d_batch_res = nullptr;
     uint64_t batch_res_cap = 0;
+    sjb::BatchTune tune;
+    struct sjb_batch_plan *scratch_plan = nullptr;  // reused by sjb_lazy_join_batch
 
     // NLJ bitmap path
     uint64_t *d_masks = nullptr;      // (left idx, word idx, mask) triples
@@ -159,7 +202,7 @@ int dev_alloc(sjb_ctx *c, void **p, size_t bytes);
 int host_alloc(sjb_ctx *c, void **p, size_t bytes);
 
 // ---- launchers (partition.hip) ----
-int choose_logF(int alg, uint64_t n_build, int requested);
+int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested);
 int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
                   int logF);
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
@@ -192,6 +235,17 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
 int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const uint64_t *loff,
                const sjb_tuple_t *R, const uint64_t *roff, uint32_t n_windows,
                sjb_window_result_t *results);
+void batch_plan_free(struct sjb_batch_plan *p);
+int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias);
+// batched kernels (partition.hip / join.hip)
+int launch_partition_batch(hipStream_t s, int kind, const BatchArgs &a, uint32_t n_chunks,
+                           size_t dyn_bytes);
+int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a, int blocks,
+                      size_t dyn_bytes);
+int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead);
+size_t partition_lds_bytes_chunk(int logF, int chunk);
+size_t join_view_bytes(int alg, int ncL, int ncR);
+int join_idcap(int alg);
 
 // ---- pointer checks / output staging (api.hip) ----
 // True when the device can dereference p (device memory, or pinned /

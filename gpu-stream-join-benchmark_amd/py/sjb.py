@@ -22,7 +22,7 @@ DIST_UNIFORM, DIST_ZIPF = 0, 1
 E = {
     0: "SJB_OK", -1: "SJB_E_INVALID", -2: "SJB_E_CAPACITY", -3: "SJB_E_NOMEM", -4: "SJB_E_HIP",
     -5: "SJB_E_UNSUPPORTED", -6: "SJB_E_NODEVICE", -7: "SJB_E_BUDGET", -8: "SJB_E_TIMEOUT",
-    -9: "SJB_E_STATE",
+    -9: "SJB_E_STATE", -10: "SJB_E_MISMATCH",
 }
 
 TUPLE_DTYPE = np.dtype([("key", "<u4"), ("val", "<u4"), ("ts", "<u8")])
@@ -54,6 +54,20 @@ class Options(ctypes.Structure):
                 ("fanout_log2", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
+class BatchTune(ctypes.Structure):
+    _fields_ = [("fanout_bias", ctypes.c_int32), ("chunk_log2", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("grab", ctypes.c_uint32), ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_windows", ctypes.c_uint32), ("n_chunks", ctypes.c_uint32),
+                ("n_items", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("tune", BatchTune)]
+
+
 # exported symbols and their signatures: (restype, argtypes)
 _u64, _u32, _i, _p, _d = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
 SIGNATURES = {
@@ -69,6 +83,14 @@ SIGNATURES = {
                               ctypes.POINTER(Stats)]),
     "sjb_lazy_join_async": (_i, [_p, _i, _u32, _p, _u64, _p, _u64, _p]),
     "sjb_lazy_join_batch": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _p]),
+    "sjb_batch_plan_create": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _p, ctypes.POINTER(_p)]),
+    "sjb_batch_plan_launch": (_i, [_p]),
+    "sjb_batch_plan_info": (_i, [_p, ctypes.POINTER(BatchInfo)]),
+    "sjb_batch_plan_destroy": (_i, [_p]),
+    "sjb_batch_autotune": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _u32, ctypes.POINTER(BatchTune),
+                                ctypes.POINTER(_u32)]),
+    "sjb_batch_get_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
+    "sjb_batch_set_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
     "sjb_eager_execute": (_i, [_p, _i, _i, _u32, _p, _u64, _i, _p, _u64, ctypes.POINTER(Stats)]),
     "sjb_eager_clear": (_i, [_p]),
     "sjb_eager_end_of_stream": (_i, [_p]),
@@ -130,11 +152,15 @@ class Context:
                  hbm_budget=0, fanout_log2=0, flags=0):
         o = Options(max_window, distinct_keys, batch_size, hbm_budget, fanout_log2, flags)
         h = ctypes.c_void_p()
+        self._plans = []
         check(lib().sjb_open(device, ctypes.byref(o), ctypes.byref(h)), "sjb_open")
         self.h = h
 
     def close(self):
         if self.h:
+            for plan in self._plans:
+                plan.destroy()
+            self._plans = []
             lib().sjb_close(self.h)
             self.h = None
 
@@ -192,6 +218,38 @@ class Context:
                                         nw, ptr(res)), "sjb_lazy_join_batch")
         return res
 
+    def batch_plan(self, alg, flags, left, left_off, right, right_off, d_results):
+        """Creates a captured batch plan; d_results: device array of len(left_off)-1
+        sjb_window_result_t (32 B each)."""
+        lo = np.ascontiguousarray(left_off, dtype=np.uint64)
+        ro = np.ascontiguousarray(right_off, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        check(lib().sjb_batch_plan_create(self.h, alg, flags, ptr(left), ptr(lo), ptr(right), ptr(ro),
+                                          len(lo) - 1, ptr(d_results), ctypes.byref(h)),
+              "sjb_batch_plan_create")
+        plan = BatchPlan(h, self)
+        self._plans.append(plan)
+        return plan
+
+    def batch_autotune(self, alg, flags, left, left_off, right, right_off, reps=3):
+        lo = np.ascontiguousarray(left_off, dtype=np.uint64)
+        ro = np.ascontiguousarray(right_off, dtype=np.uint64)
+        best, tried = BatchTune(), ctypes.c_uint32()
+        check(lib().sjb_batch_autotune(self.h, alg, flags, ptr(left), ptr(lo), ptr(right), ptr(ro),
+                                       len(lo) - 1, reps, ctypes.byref(best), ctypes.byref(tried)),
+              "sjb_batch_autotune")
+        return best, tried.value
+
+    def batch_tune(self, **kw):
+        """Returns the ctx's batch launch shape; keyword arguments set fields first."""
+        t = BatchTune()
+        check(lib().sjb_batch_get_tune(self.h, ctypes.byref(t)), "sjb_batch_get_tune")
+        if kw:
+            for k, v in kw.items():
+                setattr(t, k, v)
+            check(lib().sjb_batch_set_tune(self.h, ctypes.byref(t)), "sjb_batch_set_tune")
+        return t
+
     def eager_execute(self, alg, method, flags, batch, is_left, out=None, out_cap=0):
         st = Stats()
         rc = lib().sjb_eager_execute(self.h, alg, method, flags, ptr(batch), len(batch),
@@ -214,3 +272,32 @@ class Context:
 
     def pk_launches(self):
         return lib().sjb_pk_launches(self.h)
+
+
+class BatchPlan:
+    """A captured batch of windows (sjb_batch_plan_*): launch() replays the graph."""
+
+    def __init__(self, h, ctx):
+        self.h = h
+        self.ctx = ctx  # plans are destroyed before their ctx closes
+
+    def launch(self):
+        if not self.h:
+            raise SjbError(-1, "sjb_batch_plan_launch (destroyed plan)")
+        check(lib().sjb_batch_plan_launch(self.h), "sjb_batch_plan_launch")
+
+    def info(self):
+        i = BatchInfo()
+        check(lib().sjb_batch_plan_info(self.h, ctypes.byref(i)), "sjb_batch_plan_info")
+        return i
+
+    def destroy(self):
+        if self.h:
+            lib().sjb_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -43,6 +43,7 @@ extern "C" {
 #define SJB_E_BUDGET (-7)      /* HBM budget would be exceeded                 */
 #define SJB_E_TIMEOUT (-8)     /* a bounded wait (persistent kernel) expired   */
 #define SJB_E_STATE (-9)       /* call not valid in the current ctx state      */
+#define SJB_E_MISMATCH (-10)   /* autotuner: two launch shapes disagreed       */
 
 /* ---- enums: same values as include/sjb/windowing/BaseWindowing.cuh:9-40 - */
 /* ALGORITHM_TYPE */
@@ -185,6 +186,57 @@ int sjb_lazy_join_batch(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
                         const sjb_tuple_t *d_left, const uint64_t *left_off,
                         const sjb_tuple_t *d_right, const uint64_t *right_off,
                         uint32_t n_windows, sjb_window_result_t *results);
+
+/* ---- batched windows: plans, hipGraph replay, autotuning --------------- */
+/* A plan fixes a batch layout (the windows of sjb_lazy_join_batch over fixed
+ * device streams d_left / d_right), owns the partition buffers, uploads the
+ * window descriptors once and captures the batch's launches (clear,
+ * partition, persistent join) in a hipGraph.  sjb_batch_plan_launch replays
+ * the graph on the ctx stream (asynchronous); per-window results land in
+ * the caller's device array d_results[n_windows] (HJ / SMJ; count, C1 and,
+ * with SJB_F_C2, C2).  This replaces the reference's one-executor-call-per-
+ * window loop (include/sjb/windowing/lazy/CountBased.cuh:103-166) for a
+ * known window schedule; it has no single reference counterpart.           */
+typedef struct sjb_batch_plan sjb_batch_plan_t;
+
+/* Launch shape of the batched path (the autotuner's search space).        */
+typedef struct sjb_batch_tune {
+    int32_t fanout_bias;    /* added to every window's log2 partition fanout */
+    uint32_t chunk_log2;    /* partition-pass chunk: 12 or 14                */
+    uint32_t blocks_per_cu; /* resident persistent join workgroups per CU    */
+    uint32_t grab;          /* join work items taken per queue access        */
+    double ms;              /* autotune: median batch time of this shape     */
+} sjb_batch_tune_t;
+
+typedef struct sjb_batch_info {
+    uint32_t n_windows;
+    uint32_t n_chunks;      /* partition-pass work items                     */
+    uint32_t n_items;       /* join work items (window partitions)           */
+    uint32_t blocks;        /* persistent join workgroups                    */
+    uint64_t device_bytes;  /* device memory held by the plan                */
+    sjb_batch_tune_t tune;
+} sjb_batch_info_t;
+
+int sjb_batch_plan_create(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
+                          const sjb_tuple_t *d_left, const uint64_t *left_off,
+                          const sjb_tuple_t *d_right, const uint64_t *right_off,
+                          uint32_t n_windows, sjb_window_result_t *d_results,
+                          sjb_batch_plan_t **plan);
+int sjb_batch_plan_launch(sjb_batch_plan_t *plan);
+int sjb_batch_plan_info(sjb_batch_plan_t *plan, sjb_batch_info_t *info);
+int sjb_batch_plan_destroy(sjb_batch_plan_t *plan);
+
+/* Tries every launch shape of the search space on the given batch, checks
+ * that all of them give bit-identical per-window results (SJB_E_MISMATCH
+ * otherwise), stores the fastest in the ctx (used by later batched calls
+ * and plans) and returns it in *best.                                       */
+int sjb_batch_autotune(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
+                       const sjb_tuple_t *d_left, const uint64_t *left_off,
+                       const sjb_tuple_t *d_right, const uint64_t *right_off,
+                       uint32_t n_windows, uint32_t reps, sjb_batch_tune_t *best,
+                       uint32_t *n_tried);
+int sjb_batch_get_tune(sjb_ctx_t *ctx, sjb_batch_tune_t *tune);
+int sjb_batch_set_tune(sjb_ctx_t *ctx, const sjb_batch_tune_t *tune);
 
 /* ---- eager (per-batch) join -------------------------------------------- */
 /* Joins one incoming batch against the opposite side's resident window state

@@ -50,9 +50,10 @@ def test_skewed_keys(ctx, torch_mod, orc, alg, theta):
     R = make(n, 1, 11, keys=zipf_keys(n, theta, 1 << 14, 1))
     S = make(n, 1, 12, keys=zipf_keys(n, theta, 1 << 14, 2))
     o, _ = orc.lazy_smj(R, S)
-    flags = F_GROUPED if alg == SMJ else 0
-    st = run_count(ctx, torch_mod, alg, R, S, flags)
-    assert (st.result_count, st.checksum1) == (o.count, o.c1)
+    for flags in (0, F_GROUPED):
+        st = run_count(ctx, torch_mod, alg, R, S, flags)
+        assert (st.result_count, st.checksum1) == (o.count, o.c1)
+        assert st.c2_valid == 0
 
 
 def test_grouped_equals_pairwise(ctx, torch_mod):
@@ -60,8 +61,21 @@ def test_grouped_equals_pairwise(ctx, torch_mod):
     a = run_count(ctx, torch_mod, SMJ, R, S, F_GROUPED)
     b = run_count(ctx, torch_mod, SMJ, R, S, F_C2)
     c = run_count(ctx, torch_mod, HJ, R, S, F_C2)
+    d = run_count(ctx, torch_mod, HJ, R, S, F_GROUPED)
     assert (a.result_count, a.checksum1) == (b.result_count, b.checksum1)
+    assert (d.result_count, d.checksum1) == (b.result_count, b.checksum1)
     assert (b.result_count, b.checksum1, b.checksum2) == (c.result_count, c.checksum1, c.checksum2)
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_grouped_key_ffffffff(ctx, torch_mod, orc, alg):
+    # the aggregation table's empty marker is a legal key
+    R, S = make(5000, 40, 141), make(4000, 40, 142)
+    R["key"][::3] = 0xFFFFFFFF
+    S["key"][::5] = 0xFFFFFFFF
+    o, _ = orc.lazy_smj(R, S)
+    st = run_count(ctx, torch_mod, alg, R, S, F_GROUPED)
+    assert (st.result_count, st.checksum1) == (o.count, o.c1)
 
 
 @pytest.mark.parametrize("alg,method", [(HJ, CK), (HJ, ES), (HJ, AT), (SMJ, CK), (SMJ, ES),
