@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 
 #include "sjb_device.hpp"
@@ -90,6 +91,37 @@ int out_finish(sjb_ctx *c, const OutBuf &ob, uint64_t rows) {
 static bool valid_alg(int alg) { return alg == SJB_ALG_NLJ || alg == SJB_ALG_HJ || alg == SJB_ALG_SMJ; }
 static bool valid_method(int m) { return m >= SJB_OUT_NOOUTPUT && m <= SJB_OUT_ATOMIC; }
 
+// Plans one HJ/SMJ window (fanout, partition function) and enqueues its
+// partition pass (plus the range parameters or the heavy-key reset it
+// needs).  Both sides non-empty.
+int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out) {
+    WindowPlan &p = *out;
+    p.alg = alg;
+    p.flags = flags;
+    p.nL = nL;
+    p.nR = nR;
+    p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2));
+    p.kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
+             : alg == SJB_ALG_SMJ                     ? PART_RANGE
+                                                      : PART_HASH;
+    p.payload = write ? PAY_IDX : PAY_VAL;
+    p.peel = p.kind == PART_LEARNED && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
+    p.ncL = int((nL + kChunk - 1) / kChunk);
+    p.ncR = int((nR + kChunk - 1) / kChunk);
+    if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
+    int rc;
+    if (p.kind == PART_RANGE) {
+        rc = launch_minmax(c, L, nL, R, nR, p.logF);
+        if (rc) return rc;
+    }
+    if (p.peel) {
+        rc = launch_fill_u64(c->stream, c->d_heavy, 0, 4 * kMaxHeavy);
+        if (rc) return rc;
+    }
+    return launch_partition(c, p, L, R);
+}
+
 // Enqueues one window (count/checksums or pair output) on the ctx stream;
 // the result lands in d_res.  write: 0 = count, 1 = materialise.
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
@@ -116,30 +148,14 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
         return SJB_OK;
     }
     WindowPlan p;
-    p.alg = alg;
-    p.flags = flags;
-    p.nL = nL;
-    p.nR = nR;
-    p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2));
-    p.kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
-             : alg == SJB_ALG_SMJ                     ? PART_RANGE
-                                                      : PART_HASH;
-    p.payload = write ? PAY_IDX : PAY_VAL;
-    p.ncL = int((nL + kChunk - 1) / kChunk);
-    p.ncR = int((nR + kChunk - 1) / kChunk);
-    if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
-    *logF_used = p.logF;
-    if (p.kind == PART_RANGE) {
-        rc = launch_minmax(c, L, nL, R, nR, p.logF);
-        if (rc) return rc;
-    }
-    rc = launch_partition(c, p, L, R);
+    rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p);
     if (rc) return rc;
+    *logF_used = p.logF;
     if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res);
+    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res, p.peel ? c->rmi_nheavy : 0);
     if (rc) return rc;
     if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
     return SJB_OK;
@@ -327,6 +343,7 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
         (rc = dev_alloc(c, (void **)&c->d_result, 64 * sizeof(sjb_window_result_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_pparams, sizeof(PartParams))) ||
         (rc = dev_alloc(c, (void **)&c->d_rmi, sizeof(RmiParams))) ||
+        (rc = dev_alloc(c, (void **)&c->d_heavy, 4 * kMaxHeavy * sizeof(uint64_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_cursor, 4 * sizeof(uint64_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_masks, 3 * c->mask_cap * sizeof(uint64_t))) ||
         (rc = host_alloc(c, (void **)&c->h_result, 64 * sizeof(sjb_window_result_t))))
@@ -419,6 +436,50 @@ int sjb_lazy_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
     if (rc) return rc;
     return lazy_join_device(c, algorithm, method, flags, c->d_stage[0], nL, c->d_stage[1], nR, out,
                             out_cap, stats);
+}
+
+int sjb_partition_report(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_tuple_t *d_left,
+                         uint64_t n_left, const sjb_tuple_t *d_right, uint64_t n_right,
+                         sjb_partition_report_t *rep) {
+    if (!c || !rep || (algorithm != SJB_ALG_HJ && algorithm != SJB_ALG_SMJ)) return SJB_E_INVALID;
+    if (n_left == 0 || n_right == 0 || n_left > c->cap || n_right > c->cap) return SJB_E_INVALID;
+    if (!device_accessible(d_left) || !device_accessible(d_right)) return SJB_E_INVALID;
+    SJB_HIP_TRY(hipSetDevice(c->device));
+    flags |= c->opts.flags;
+    WindowPlan p;
+    int rc = window_partition(c, algorithm, flags, d_left, n_left, d_right, n_right, 0, &p);
+    if (rc) return rc;
+    const int F = 1 << p.logF;
+    uint32_t *d_sizes = reinterpret_cast<uint32_t *>(c->d_partials);  // 2F u32
+    rc = launch_part_sizes(c->stream, c->d_offs, p.logF, p.ncL, p.ncR, d_sizes);
+    if (rc) return rc;
+    std::vector<uint32_t> sz(2 * size_t(F));
+    uint64_t heavy[4 * kMaxHeavy] = {};
+    SJB_HIP_TRY(hipMemcpyAsync(sz.data(), d_sizes, sz.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    if (p.peel)
+        SJB_HIP_TRY(hipMemcpyAsync(heavy, c->d_heavy, sizeof(heavy), hipMemcpyDeviceToHost, c->stream));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t cap = algorithm == SJB_ALG_SMJ ? uint64_t(kSmjCap) : uint64_t(kHjCap);
+    uint64_t tot = 0, over = 0, mxb = 0, mxp = 0;
+    for (int i = 0; i < F; i++) {
+        tot += sz[i];
+        mxb = std::max<uint64_t>(mxb, sz[i]);
+        mxp = std::max<uint64_t>(mxp, sz[F + i]);
+        if (sz[i] > cap) over += sz[i];
+    }
+    std::memset(rep, 0, sizeof(*rep));
+    rep->fanout_log2 = uint32_t(p.logF);
+    rep->kind = uint32_t(p.kind);
+    rep->max_build = mxb;
+    rep->max_probe = mxp;
+    rep->mean_build = double(tot) / F;
+    rep->imbalance = tot ? double(mxb) / rep->mean_build : 1.0;
+    rep->fixup_frac = tot ? double(over) / double(tot) : 0.0;
+    for (int h = 0; h < kMaxHeavy; h++) {
+        rep->heavy_left += heavy[4 * h];
+        rep->heavy_right += heavy[4 * h + 2];
+    }
+    return SJB_OK;
 }
 
 int sjb_lazy_join_async(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_tuple_t *d_left,

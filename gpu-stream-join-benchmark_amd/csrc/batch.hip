@@ -37,6 +37,10 @@ struct sjb_batch_plan {
     sjb::Rec *d_part = nullptr;
     uint32_t *d_offs = nullptr;
     uint32_t *d_qhead = nullptr;
+    uint64_t *d_heavy = nullptr;      // learned + grouped: per-window heavy-key sums
+    size_t cap_heavy = 0;
+    bool peel = false;
+    uint32_t n_heavy = 0;
     size_t cap_win = 0, cap_chunks = 0, cap_items = 0, cap_part = 0, cap_offs = 0;
     uint64_t dev_bytes = 0;
     // pinned staging of the descriptors
@@ -78,7 +82,7 @@ void batch_plan_free(sjb_batch_plan *p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *dev[] = {p->d_win, p->d_chunk_win, p->d_item_win, p->d_part, p->d_offs, p->d_qhead};
+    void *dev[] = {p->d_win, p->d_chunk_win, p->d_item_win, p->d_part, p->d_offs, p->d_qhead, p->d_heavy};
     for (void *d : dev)
         if (d) (void)hipFree(d);
     if (p->h_stage) (void)hipHostFree(p->h_stage);
@@ -188,6 +192,7 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
         size_t cap = 0;
         if ((rc = grow(p, &p->d_qhead, &cap, kXcds * 16))) return rc;
     }
+    if (p->peel && (rc = grow(p, &p->d_heavy, &p->cap_heavy, size_t(n) * kMaxHeavy * 4))) return rc;
 
     // stage descriptors + maps in pinned memory, one H2D copy each
     const size_t bytes_win = size_t(n) * sizeof(BatchWin);
@@ -232,6 +237,7 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     a.grab = uint32_t(std::max(p->tune.grab, 1));
     a.idcap = uint32_t(join_idcap(p->alg));
     a.rmi = c->d_rmi;
+    a.heavy = p->peel ? p->d_heavy : nullptr;
     p->dyn_part = partition_lds_bytes_chunk(max_logF, int(chunk));
     p->dyn_join = join_view_bytes(p->alg, max_ncL, max_ncR);
     p->blocks = int(std::min<uint64_t>(uint64_t(c->n_cu) * std::max(p->tune.blocks_per_cu, 1),
@@ -240,12 +246,16 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
 }
 
 static int plan_enqueue(sjb_batch_plan *p, hipStream_t s) {
-    int rc = launch_batch_clear(s, p->d_res, p->n_win, p->d_qhead);
+    int rc = launch_batch_clear(s, p->d_res, p->n_win, p->d_qhead, p->peel ? p->d_heavy : nullptr);
     if (rc) return rc;
-    rc = launch_partition_batch(s, p->kind, p->args, p->n_chunks, p->dyn_part);
+    rc = launch_partition_batch(s, p->kind, p->peel, p->args, p->n_chunks, p->dyn_part);
     if (rc) return rc;
-    if (p->n_items == 0) return SJB_OK;
-    return launch_join_batch(s, p->alg, p->flags, p->args, p->blocks, p->dyn_join);
+    if (p->n_items) {
+        rc = launch_join_batch(s, p->alg, p->flags, p->args, p->blocks, p->dyn_join);
+        if (rc) return rc;
+    }
+    if (p->peel) return launch_heavy_finalize(s, p->d_res, p->n_win, p->d_heavy, p->n_heavy);
+    return SJB_OK;
 }
 
 static int plan_capture(sjb_batch_plan *p) {
@@ -275,6 +285,8 @@ static int plan_init(sjb_ctx *c, sjb_batch_plan *p, int alg, uint32_t flags, uin
     p->kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
               : alg == SJB_ALG_SMJ                     ? PART_RANGE
                                                        : PART_HASH;
+    p->peel = p->kind == PART_LEARNED && (flags & SJB_F_GROUPED) && c->rmi_nheavy > 0;
+    p->n_heavy = p->peel ? c->rmi_nheavy : 0;
     p->tune = c->tune;
     p->n_win = n;
     p->d_res = d_res;

@@ -132,3 +132,53 @@ extern "C" int sjb_generate(sjb_tuple_t *out, uint64_t n, int dist, uint64_t key
     for (auto &x : th) x.join();
     return SJB_OK;
 }
+
+extern "C" int sjb_generate_mix(sjb_tuple_t *out, uint64_t n, uint64_t window, double zipf_frac,
+                                double theta, uint64_t seed, uint64_t ts_base) {
+    if ((n && !out) || !(zipf_frac >= 0.0 && zipf_frac <= 1.0) || !(theta > 0.0)) return SJB_E_INVALID;
+    constexpr int kComp = 8;
+    const double two32 = 4294967296.0;
+    double cum[kComp], mean[kComp], sigma[kComp];
+    double wsum = 0;
+    for (int j = 0; j < kComp; j++) wsum += 1 + (5 * j) % 4;
+    double acc = 0;
+    const double drift = 0.01 * two32 * double(window / 64);
+    for (int j = 0; j < kComp; j++) {
+        acc += (1 + (5 * j) % 4) / wsum;
+        cum[j] = acc;
+        mean[j] = fmod((0.06 + 0.12 * j) * two32 + drift, two32);
+        sigma[j] = (0.004 + 0.002 * (j % 4)) * two32;
+    }
+    const Zipf z(theta, 1ULL << 32);
+    const Scramble sc(1ULL << 32);
+    auto work = [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) {
+            uint64_t key;
+            if (u01(draw(seed, i, 2)) < zipf_frac) {
+                uint64_t d = 8;
+                key = sc(z.sample([&] { return u01(draw(seed, i, d++)); }) - 1);
+            } else {
+                const double c = u01(draw(seed, i, 3));
+                int j = 0;
+                while (j < kComp - 1 && c >= cum[j]) j++;
+                // Box-Muller from two further draws
+                const double u1 = (double(draw(seed, i, 4) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                const double u2 = u01(draw(seed, i, 5));
+                const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+                double v = fmod(floor(mean[j] + sigma[j] * g + 0.5), two32);
+                if (v < 0) v += two32;
+                key = uint64_t(v) & 0xFFFFFFFFULL;
+            }
+            out[i].key = uint32_t(key);
+            out[i].val = uint32_t(draw(seed, i, 1));
+            out[i].ts = ts_base + i;
+        }
+    };
+    unsigned hw = std::thread::hardware_concurrency();
+    unsigned T = std::max(1u, std::min(hw ? hw : 1u, 16u));
+    if (n < (1u << 16)) T = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) th.emplace_back(work, n * t / T, n * (t + 1) / T);
+    for (auto &x : th) x.join();
+    return SJB_OK;
+}

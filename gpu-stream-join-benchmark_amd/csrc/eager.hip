@@ -207,7 +207,7 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
                 c->d_batch, n, c->d_eager_tuples[other], c->d_eager_next[other], c->d_eager_head[other],
                 uint32_t(c->eager_nb - 1), is_left, c->d_partials, nullptr, 0, 0, cur);
             SJB_HIP_TRY(hipGetLastError());
-            rc = launch_reduce(c, int(blocks), c->d_result);
+            rc = launch_reduce(c, int(blocks), c->d_result, 0);
         } else {
             const sjb_tuple_t *L0 = is_left ? c->d_batch : c->d_eager_tuples[other];
             const sjb_tuple_t *R0 = is_left ? c->d_eager_tuples[other] : c->d_batch;
@@ -243,7 +243,7 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
             SJB_EP(false, false);
 #undef SJB_EP
         SJB_HIP_TRY(hipGetLastError());
-        rc = launch_reduce(c, int(blocks), c->d_result);
+        rc = launch_reduce(c, int(blocks), c->d_result, 0);
         if (rc) return rc;
     } else {
         // NLJ: incoming batch x the other side's resident window

@@ -901,7 +901,10 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_batch(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // partial reduction -> one sjb_window_result_t
 // ---------------------------------------------------------------------------
+// n_heavy > 0: also adds the peeled heavy keys' pairs (count cntL * cntR,
+// C1 sumA * sumB per key; 4 u64 per key in heavy[]).
 __global__ __launch_bounds__(1024) void k_reduce(const uint64_t *__restrict__ partials, int n,
+                                                 const uint64_t *__restrict__ heavy, uint32_t n_heavy,
                                                  sjb_window_result_t *__restrict__ out) {
     __shared__ uint64_t s_red[3 * 1024 / kWave];
     Acc a;
@@ -926,6 +929,11 @@ __global__ __launch_bounds__(1024) void k_reduce(const uint64_t *__restrict__ pa
             a.c2 += v[u][2];
         }
     }
+    if (threadIdx.x < n_heavy) {
+        const uint64_t *h = heavy + 4 * threadIdx.x;
+        a.count += h[0] * h[2];
+        a.c1 += h[1] * h[3];
+    }
     const Acc r = block_sum(a, s_red);
     if (threadIdx.x == 0) {
         out->count = r.count;
@@ -935,8 +943,8 @@ __global__ __launch_bounds__(1024) void k_reduce(const uint64_t *__restrict__ pa
     }
 }
 
-int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out) {
-    k_reduce<<<1, 1024, 0, c->stream>>>(c->d_partials, n_partials, d_out);
+int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy) {
+    k_reduce<<<1, 1024, 0, c->stream>>>(c->d_partials, n_partials, c->d_heavy, n_heavy, d_out);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }

@@ -178,7 +178,7 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
                                                                      c->d_partials, out, out_cap, wrap, cur);
     }
     SJB_HIP_TRY(hipGetLastError());
-    return launch_reduce(c, int(gx * gy), d_out);
+    return launch_reduce(c, int(gx * gy), d_out, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,7 +305,7 @@ int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_t
     k_bitmap_decode<<<blocks, 256, 0, c->stream>>>(c->d_masks, cnt, L, R, out, out_cap, cur,
                                                    c->d_partials);
     SJB_HIP_TRY(hipGetLastError());
-    return launch_reduce(c, blocks, d_out);
+    return launch_reduce(c, blocks, d_out, 0);
 }
 
 }  // namespace sjb

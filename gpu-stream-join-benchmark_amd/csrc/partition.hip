@@ -22,6 +22,63 @@ namespace sjb {
 // (bijective, so equal keys and only equal keys collide before truncation).
 // RANGE: (key - kmin) >> shift, i.e. MSD radix on the key digits above the
 // window's minimum.  LEARNED: floor(F * cdf(key)) from the RMI.
+// Learned partition of key k into F parts (RMI of RmiParams; fminf/fmaxf
+// also map a NaN prediction to 0).
+__device__ __forceinline__ uint32_t rmi_partition(const RmiParams *__restrict__ m, uint32_t k, uint32_t F) {
+    const float x = k < m->kmin ? 0.f : float(k - m->kmin) * m->scale;
+    float r;
+    if (m->root_kind == ROOT_LINEAR) {
+        r = fmaf(m->root_a, x, m->root_b);
+    } else {
+        const float u = fminf(fmaxf(x, 0.f), 1.f) * float(kRootTable);
+        const int i = min(int(u), kRootTable - 1);
+        r = fmaf(m->table[i + 1] - m->table[i], u - float(i), m->table[i]);
+    }
+    r = fminf(fmaxf(r * float(kRmiLeaves), 0.f), float(kRmiLeaves - 1));
+    const int leaf = int(r);
+    float c = fmaf(m->leaf_a[leaf], x, m->leaf_b[leaf]);
+    c = fminf(fmaxf(c * float(F), 0.f), float(F - 1));
+    return uint32_t(c);
+}
+
+// Heavy-key set of a learned model in LDS (64-slot open addressing):
+// slot -> key, index in RmiParams::heavy (-1 = empty).
+struct HeavySet {
+    uint32_t key[64];
+    int idx[64];
+    uint32_t cnt[kMaxHeavy];
+    uint64_t sum[kMaxHeavy];
+};
+
+// Thread 0 fills the set; the caller puts a barrier after it.
+__device__ __forceinline__ void heavy_set_load(HeavySet &h, const RmiParams *__restrict__ m) {
+    if (threadIdx.x < 64) h.idx[threadIdx.x] = -1;
+    if (threadIdx.x < kMaxHeavy) {
+        h.cnt[threadIdx.x] = 0;
+        h.sum[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < m->n_heavy && i < uint32_t(kMaxHeavy); i++) {
+            const uint32_t k = m->heavy[i];
+            uint32_t s = (k * kHashMul) >> 26;
+            while (h.idx[s] >= 0 && h.key[s] != k) s = (s + 1) & 63;
+            h.key[s] = k;
+            h.idx[s] = int(i);
+        }
+    }
+}
+
+__device__ __forceinline__ int heavy_find(const HeavySet &h, uint32_t k) {
+    uint32_t s = (k * kHashMul) >> 26;
+    for (;;) {
+        const int i = h.idx[s];
+        if (i < 0) return -1;
+        if (h.key[s] == k) return i;
+        s = (s + 1) & 63;
+    }
+}
+
 template <int KIND>
 struct PartFn {
     uint32_t logF, fmask, kmin, shift;
@@ -50,14 +107,7 @@ struct PartFn {
             return p > fmask ? fmask : p;
         }
         // LEARNED
-        float x = float(k);
-        float l = fmaf(rmi->root_a, x, rmi->root_b);
-        int leaf = int(l);
-        leaf = leaf < 0 ? 0 : (leaf >= kRmiLeaves ? kRmiLeaves - 1 : leaf);
-        float cdf = fmaf(rmi->leaf_a[leaf], x, rmi->leaf_b[leaf]);
-        int p = int(cdf * float(fmask + 1));
-        p = p < 0 ? 0 : (p > int(fmask) ? int(fmask) : p);
-        return uint32_t(p);
+        return rmi_partition(rmi, k, fmask + 1);
     }
 };
 
@@ -138,17 +188,25 @@ int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple
 // is a full, coalesced line.  The chunk's local partition offsets (F+1
 // entries) go to `o`.  The join gathers partition p as the run
 // [o_c[p], o_c[p+1]) of every chunk c.
-template <int KIND, int PAY, int CHUNK>
+//
+// PEEL (learned model, grouped mode): records whose key is in the model's
+// heavy set go to an extra bucket F after the last partition — they are
+// not written back (o[F] ends partition F-1) — and are aggregated instead:
+// per heavy key the count and the sum of A (left side) or B (right side)
+// are added to heavy_acc[4*h + (left ? 0 : 2) + {0,1}].
+template <int KIND, int PAY, int CHUNK, bool PEEL>
 __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ in, uint64_t base,
                                                 int len, int logF, const PartFn<KIND> &part,
                                                 uint32_t *__restrict__ o, Rec *__restrict__ out,
-                                                uint32_t *lds) {
+                                                uint32_t *lds, HeavySet *hs, bool left,
+                                                uint64_t *__restrict__ heavy_acc) {
     const int F = 1 << logF;
-    uint32_t *hist = lds;                                  // F + 1 counters
-    uint32_t *ws = lds + F + 1;                            // wave sums
-    Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 1 + kPartThreads / kWave + 1) & ~1));
+    const int NB = F + (PEEL ? 1 : 0);                     // buckets
+    uint32_t *hist = lds;                                  // NB + 1 counters
+    uint32_t *ws = lds + F + 2;                            // wave sums
+    Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 2 + kPartThreads / kWave + 1) & ~1));
 
-    for (int q = threadIdx.x; q <= F; q += kPartThreads) hist[q] = 0;
+    for (int q = threadIdx.x; q <= NB; q += kPartThreads) hist[q] = 0;
     __syncthreads();
     constexpr int kPer = CHUNK / kPartThreads;
     Rec rec[kPer];
@@ -165,30 +223,42 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
     for (int j = 0; j < kPer; j++) {
         const int e = j * kPartThreads + threadIdx.x;
         if (e < len) {
-            pid[j] = part(rec[j].x);
+            if (PEEL) {
+                const int h = heavy_find(*hs, rec[j].x);
+                if (h >= 0) {
+                    pid[j] = uint32_t(F);
+                    atomicAdd(&hs->cnt[h], 1u);
+                    const uint64_t v = left ? hash_a(rec[j].x, rec[j].y) : hash_b(rec[j].x, rec[j].y);
+                    atomicAdd(reinterpret_cast<unsigned long long *>(&hs->sum[h]), (unsigned long long)v);
+                } else {
+                    pid[j] = part(rec[j].x);
+                }
+            } else {
+                pid[j] = part(rec[j].x);
+            }
             rank[j] = atomicAdd(&hist[pid[j]], 1u);
         }
     }
     __syncthreads();
-    // exclusive scan of hist[0..F) in place; hist[F] = len
+    // exclusive scan of hist[0..NB) in place; hist[NB] = len
     {
-        constexpr int kMaxPer = (1 << kMaxLogF) / kPartThreads;
+        constexpr int kMaxPer = ((1 << kMaxLogF) + 1 + kPartThreads - 1) / kPartThreads;
         uint32_t v[kMaxPer];
         uint32_t sum = 0;
-        const int per = (F + kPartThreads - 1) / kPartThreads;
+        const int per = (NB + kPartThreads - 1) / kPartThreads;
         for (int q = 0; q < per && q < kMaxPer; q++) {
             const int idx = threadIdx.x * per + q;
-            v[q] = idx < F ? hist[idx] : 0;
+            v[q] = idx < NB ? hist[idx] : 0;
             sum += v[q];
         }
         uint32_t tot;
         uint32_t run = block_excl_scan_u32(sum, ws, &tot);
         for (int q = 0; q < per && q < kMaxPer; q++) {
             const int idx = threadIdx.x * per + q;
-            if (idx < F) hist[idx] = run;
+            if (idx < NB) hist[idx] = run;
             run += v[q];
         }
-        if (threadIdx.x == 0) hist[F] = tot;
+        if (threadIdx.x == 0) hist[NB] = tot;
     }
     __syncthreads();
     for (int q = threadIdx.x; q <= F; q += kPartThreads) o[q] = hist[q];
@@ -198,17 +268,28 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
         if (e < len) stage[hist[pid[j]] + rank[j]] = rec[j];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < len; e += kPartThreads) out[e] = stage[e];
+    const int keep = PEEL ? int(hist[F]) : len;
+    for (int e = threadIdx.x; e < keep; e += kPartThreads) out[e] = stage[e];
+    if (PEEL && threadIdx.x < kMaxHeavy && hs->cnt[threadIdx.x]) {
+        uint64_t *a = heavy_acc + 4 * threadIdx.x + (left ? 0 : 2);
+        atomicAdd(reinterpret_cast<unsigned long long *>(a), (unsigned long long)hs->cnt[threadIdx.x]);
+        atomicAdd(reinterpret_cast<unsigned long long *>(a + 1), (unsigned long long)hs->sum[threadIdx.x]);
+    }
 }
 
 // One window: blocks [0, ncL) take left chunks and [ncL, ncL+ncR) right
 // chunks; chunk b's offsets row is offs[b * (F+1)].
-template <int KIND, int PAY>
+template <int KIND, int PAY, bool PEEL>
 __global__ __launch_bounds__(kPartThreads) void k_partition(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
     uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, uint64_t *__restrict__ heavy_acc) {
     extern __shared__ uint32_t lds[];
+    __shared__ HeavySet hs[1];
+    if (PEEL) {
+        heavy_set_load(hs[0], rmi);
+        __syncthreads();
+    }
     PartFn<KIND> part;
     part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
     const bool left = int(blockIdx.x) < ncL;
@@ -216,9 +297,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
     const uint64_t n = left ? nL : nR;
     const uint64_t base = uint64_t(chunk) * kChunk;
     const int len = int(min<uint64_t>(kChunk, n - base));
-    partition_chunk<KIND, PAY, kChunk>(left ? L : R, base, len, logF, part,
-                                       offs + uint64_t(blockIdx.x) * ((1 << logF) + 1),
-                                       part_out + (left ? 0 : nL) + base, lds);
+    partition_chunk<KIND, PAY, kChunk, PEEL>(left ? L : R, base, len, logF, part,
+                                             offs + uint64_t(blockIdx.x) * ((1 << logF) + 1),
+                                             part_out + (left ? 0 : nL) + base, lds, hs, left,
+                                             heavy_acc);
 }
 
 // Range parameters of a batched window from a fixed strided sample of up
@@ -268,16 +350,21 @@ __device__ __forceinline__ void sample_range(const sjb_tuple_t *__restrict__ L, 
 
 // Batched windows: block b is partition-pass item b of the batch (one chunk
 // of one side of one window).
-template <int KIND, int CHUNK>
+template <int KIND, int CHUNK, bool PEEL>
 __global__ __launch_bounds__(kPartThreads) void k_partition_batch(BatchArgs a) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_mm[2];
+    __shared__ HeavySet hs[1];
     const uint32_t item = blockIdx.x;
     const BatchWin bw = a.win[a.chunk_win[item]];
     const uint32_t local = item - bw.chunk_base;
     const bool left = local < bw.ncL;
     const uint32_t chunk = left ? local : local - bw.ncL;
     const int logF = int(bw.logF);
+    if (PEEL) {
+        heavy_set_load(hs[0], a.rmi);
+        __syncthreads();
+    }
     uint32_t kmin = 0, shift = 0;
     if (KIND == PART_RANGE)
         sample_range(a.L + bw.lo, bw.nL, a.R + bw.ro, bw.nR, logF, s_mm, &kmin, &shift);
@@ -286,22 +373,42 @@ __global__ __launch_bounds__(kPartThreads) void k_partition_batch(BatchArgs a) {
     const uint32_t n = left ? bw.nL : bw.nR;
     const uint64_t base = uint64_t(chunk) * CHUNK;
     const int len = int(min<uint64_t>(CHUNK, n - base));
-    partition_chunk<KIND, PAY_VAL, CHUNK>(
+    partition_chunk<KIND, PAY_VAL, CHUNK, PEEL>(
         left ? a.L + bw.lo : a.R + bw.ro, base, len, logF, part,
         a.offs + bw.offs_base + uint64_t(local) * ((1u << logF) + 1),
-        a.part + (left ? bw.lpart : bw.rpart) + base, lds);
+        a.part + (left ? bw.lpart : bw.rpart) + base, lds, hs, left,
+        PEEL ? a.heavy + uint64_t(bw.id) * kMaxHeavy * 4 : nullptr);
+}
+
+// Adds the peeled heavy keys' pairs of every window of a batch to its
+// result: count += cntL * cntR, C1 += sumA * sumB per heavy key (C1
+// factorises per key).  One thread per window, after the join kernel.
+__global__ void k_heavy_finalize(sjb_window_result_t *__restrict__ res, uint32_t n,
+                                 const uint64_t *__restrict__ heavy, uint32_t n_heavy) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const uint64_t *h = heavy + uint64_t(w) * kMaxHeavy * 4;
+    uint64_t cnt = 0, c1 = 0;
+    for (uint32_t i = 0; i < n_heavy; i++) {
+        cnt += h[4 * i] * h[4 * i + 2];
+        c1 += h[4 * i + 1] * h[4 * i + 3];
+    }
+    res[w].count += cnt;
+    res[w].checksum1 += c1;
 }
 
 // Zeroes the per-window results and the work-queue heads of a batch (a
 // kernel, so a captured graph holds no memset node).
 __global__ void k_batch_clear(sjb_window_result_t *__restrict__ res, uint32_t n,
-                              uint32_t *__restrict__ qhead) {
+                              uint32_t *__restrict__ qhead, uint64_t *__restrict__ heavy) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         res[i].count = 0;
         res[i].checksum1 = 0;
         res[i].checksum2 = 0;
         res[i].pairs = 0;
+        if (heavy)
+            for (int j = 0; j < kMaxHeavy * 4; j++) heavy[uint64_t(i) * kMaxHeavy * 4 + j] = 0;
     }
     if (i < kXcds * 16) qhead[i] = 0;
 }
@@ -329,22 +436,22 @@ int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested) {
 }
 
 size_t partition_lds_bytes_chunk(int logF, int chunk) {
-    const size_t words = ((size_t(1) << logF) + 1 + kPartThreads / kWave + 1) & ~size_t(1);
+    const size_t words = ((size_t(1) << logF) + 2 + kPartThreads / kWave + 1) & ~size_t(1);
     return words * 4 + size_t(chunk) * sizeof(Rec);
 }
 size_t partition_lds_bytes(int logF) { return partition_lds_bytes_chunk(logF, kChunk); }
 
-template <int KIND>
+template <int KIND, bool PEEL>
 static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
                           const sjb_tuple_t *R) {
     const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;
     if (p.payload == PAY_VAL)
-        k_partition<KIND, PAY_VAL><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part);
+        k_partition<KIND, PAY_VAL, PEEL><<<nblk, kPartThreads, lds, c->stream>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part, c->d_heavy);
     else
-        k_partition<KIND, PAY_IDX><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part);
+        k_partition<KIND, PAY_IDX, false><<<nblk, kPartThreads, lds, c->stream>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part, c->d_heavy);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -353,42 +460,82 @@ int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, cons
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
     if (uint64_t((1u << p.logF) + 1) * uint64_t(p.ncL + p.ncR) > c->offs_cap) return SJB_E_CAPACITY;
     switch (p.kind) {
-        case PART_HASH: return partition_kind<PART_HASH>(c, p, L, R);
-        case PART_RANGE: return partition_kind<PART_RANGE>(c, p, L, R);
-        case PART_LEARNED: return partition_kind<PART_LEARNED>(c, p, L, R);
+        case PART_HASH: return partition_kind<PART_HASH, false>(c, p, L, R);
+        case PART_RANGE: return partition_kind<PART_RANGE, false>(c, p, L, R);
+        case PART_LEARNED:
+            return p.peel ? partition_kind<PART_LEARNED, true>(c, p, L, R)
+                          : partition_kind<PART_LEARNED, false>(c, p, L, R);
     }
     return SJB_E_INVALID;
 }
 
-template <int KIND, int CHUNK>
+template <int KIND, int CHUNK, bool PEEL>
 static int partition_batch_kind(hipStream_t s, const BatchArgs &a, uint32_t n_chunks, size_t dyn) {
-    k_partition_batch<KIND, CHUNK><<<n_chunks, kPartThreads, dyn, s>>>(a);
+    k_partition_batch<KIND, CHUNK, PEEL><<<n_chunks, kPartThreads, dyn, s>>>(a);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
 
-int launch_partition_batch(hipStream_t s, int kind, const BatchArgs &a, uint32_t n_chunks,
+int launch_partition_batch(hipStream_t s, int kind, bool peel, const BatchArgs &a, uint32_t n_chunks,
                            size_t dyn) {
     if (n_chunks == 0) return SJB_OK;
     const bool big = a.chunk == kChunk;
     if (!big && a.chunk != 4096) return SJB_E_INVALID;
     switch (kind) {
         case PART_HASH:
-            return big ? partition_batch_kind<PART_HASH, kChunk>(s, a, n_chunks, dyn)
-                       : partition_batch_kind<PART_HASH, 4096>(s, a, n_chunks, dyn);
+            return big ? partition_batch_kind<PART_HASH, kChunk, false>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_HASH, 4096, false>(s, a, n_chunks, dyn);
         case PART_RANGE:
-            return big ? partition_batch_kind<PART_RANGE, kChunk>(s, a, n_chunks, dyn)
-                       : partition_batch_kind<PART_RANGE, 4096>(s, a, n_chunks, dyn);
+            return big ? partition_batch_kind<PART_RANGE, kChunk, false>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_RANGE, 4096, false>(s, a, n_chunks, dyn);
         case PART_LEARNED:
-            return big ? partition_batch_kind<PART_LEARNED, kChunk>(s, a, n_chunks, dyn)
-                       : partition_batch_kind<PART_LEARNED, 4096>(s, a, n_chunks, dyn);
+            if (peel)
+                return big ? partition_batch_kind<PART_LEARNED, kChunk, true>(s, a, n_chunks, dyn)
+                           : partition_batch_kind<PART_LEARNED, 4096, true>(s, a, n_chunks, dyn);
+            return big ? partition_batch_kind<PART_LEARNED, kChunk, false>(s, a, n_chunks, dyn)
+                       : partition_batch_kind<PART_LEARNED, 4096, false>(s, a, n_chunks, dyn);
     }
     return SJB_E_INVALID;
 }
 
-int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead) {
+// Partition sizes of one window: sizes[p] = left records of partition p,
+// sizes[F + p] = right records (sums over the chunks' runs).
+__global__ void k_part_sizes(const uint32_t *__restrict__ offs, int logF, int ncL, int ncR,
+                             uint32_t *__restrict__ sizes) {
+    const int F = 1 << logF;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= F) return;
+    uint32_t a = 0, b = 0;
+    for (int q = 0; q < ncL + ncR; q++) {
+        const uint32_t *o = offs + uint64_t(q) * (F + 1);
+        const uint32_t len = o[p + 1] - o[p];
+        if (q < ncL)
+            a += len;
+        else
+            b += len;
+    }
+    sizes[p] = a;
+    sizes[F + p] = b;
+}
+
+int launch_part_sizes(hipStream_t s, const uint32_t *offs, int logF, int ncL, int ncR, uint32_t *sizes) {
+    const int F = 1 << logF;
+    k_part_sizes<<<(F + 255) / 256, 256, 0, s>>>(offs, logF, ncL, ncR, sizes);
+    SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
+int launch_heavy_finalize(hipStream_t s, sjb_window_result_t *res, uint32_t n, const uint64_t *heavy,
+                          uint32_t n_heavy) {
+    k_heavy_finalize<<<(n + 255) / 256, 256, 0, s>>>(res, n, heavy, n_heavy);
+    SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
+int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead,
+                       uint64_t *heavy) {
     const uint32_t m = std::max<uint32_t>(n, kXcds * 16);
-    k_batch_clear<<<(m + 255) / 256, 256, 0, s>>>(res, n, qhead);
+    k_batch_clear<<<(m + 255) / 256, 256, 0, s>>>(res, n, qhead, heavy);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -399,15 +546,21 @@ int partition_setup(sjb_ctx *c) {
     const int need = int(partition_lds_bytes(kMaxLogF));
     (void)c;
     const void *fns[] = {
-        (const void *)k_partition<PART_HASH, PAY_VAL>,    (const void *)k_partition<PART_HASH, PAY_IDX>,
-        (const void *)k_partition<PART_RANGE, PAY_VAL>,   (const void *)k_partition<PART_RANGE, PAY_IDX>,
-        (const void *)k_partition<PART_LEARNED, PAY_VAL>, (const void *)k_partition<PART_LEARNED, PAY_IDX>,
-        (const void *)k_partition_batch<PART_HASH, kChunk>,
-        (const void *)k_partition_batch<PART_RANGE, kChunk>,
-        (const void *)k_partition_batch<PART_LEARNED, kChunk>,
-        (const void *)k_partition_batch<PART_HASH, 4096>,
-        (const void *)k_partition_batch<PART_RANGE, 4096>,
-        (const void *)k_partition_batch<PART_LEARNED, 4096>,
+        (const void *)k_partition<PART_HASH, PAY_VAL, false>,
+        (const void *)k_partition<PART_HASH, PAY_IDX, false>,
+        (const void *)k_partition<PART_RANGE, PAY_VAL, false>,
+        (const void *)k_partition<PART_RANGE, PAY_IDX, false>,
+        (const void *)k_partition<PART_LEARNED, PAY_VAL, false>,
+        (const void *)k_partition<PART_LEARNED, PAY_IDX, false>,
+        (const void *)k_partition<PART_LEARNED, PAY_VAL, true>,
+        (const void *)k_partition_batch<PART_HASH, kChunk, false>,
+        (const void *)k_partition_batch<PART_RANGE, kChunk, false>,
+        (const void *)k_partition_batch<PART_LEARNED, kChunk, false>,
+        (const void *)k_partition_batch<PART_LEARNED, kChunk, true>,
+        (const void *)k_partition_batch<PART_HASH, 4096, false>,
+        (const void *)k_partition_batch<PART_RANGE, 4096, false>,
+        (const void *)k_partition_batch<PART_LEARNED, 4096, false>,
+        (const void *)k_partition_batch<PART_LEARNED, 4096, true>,
     };
     for (const void *f : fns)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, need);

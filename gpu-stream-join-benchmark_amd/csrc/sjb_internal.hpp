@@ -51,27 +51,38 @@ constexpr int kJoinThreads = SJB_JOIN_THREADS; // threads of the per-partition j
 constexpr int kHjCap = SJB_HJ_CAP;             // build entries per LDS tile (HJ)
 constexpr int kHjLogNB = SJB_HJ_LOGNB;         // log2 LDS hash buckets (HJ)
 constexpr int kSmjCap = SJB_SMJ_CAP;           // elements per LDS sort tile (SMJ)
-constexpr int kMaxHeavy = 32;        // heavy-key slots (grouped mode)
 
 enum PartKind : int { PART_HASH = 0, PART_RANGE = 1, PART_LEARNED = 2 };
 enum Payload : int { PAY_VAL = 0, PAY_IDX = 1 };
 
-// Learned CDF model (two-stage RMI): root linear over the key gives a leaf,
-// each leaf is a linear map key -> CDF position in [0,1).
+// Learned CDF model (two-stage RMI).  A key maps to x = (key - kmin) *
+// scale in [0, 1) (keys below kmin to 0); the root predicts a leaf from x
+// (linear, or the trained 1-32-32-1 MLP sampled at kRootTable+1 points and
+// interpolated), the leaf's linear model gives cdf(x), and the partition is
+// floor(F * cdf).  Keys listed in heavy[] are aggregated apart in grouped
+// mode (peeled) instead of partitioned.
 constexpr int kRmiLeaves = 1024;
+constexpr int kRootTable = 1024;
+constexpr int kMlpH = 32;
+constexpr int kMlpParams = kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;  // 1153
+constexpr int kMaxHeavy = 32;  // heavy-key slots
+enum RootKind : uint32_t { ROOT_LINEAR = 0, ROOT_TABLE = 1 };
 struct RmiParams {
-    float root_a, root_b;          // leaf = clamp(root_a * key + root_b)
-    float leaf_a[kRmiLeaves];      // cdf  = leaf_a * key + leaf_b  (key as float)
+    uint32_t kmin;
+    float scale;
+    uint32_t root_kind;
+    uint32_t n_heavy;
+    float root_a, root_b;           // linear root: leaf = (root_a * x + root_b) * kRmiLeaves
+    float table[kRootTable + 1];    // sampled MLP root at x = i / kRootTable
+    float leaf_a[kRmiLeaves];       // cdf = leaf_a * x + leaf_b
     float leaf_b[kRmiLeaves];
+    uint32_t heavy[kMaxHeavy];
 };
 
-// Device parameters of a partition function (written by a kernel or H2D).
+// Device parameters of range partitioning (written by k_minmax_final).
 struct PartParams {
-    uint32_t kmin;        // range partitioning: min key of both sides
-    uint32_t shift;       // range partitioning: key bits dropped
-    uint32_t n_heavy;     // grouped mode: number of heavy keys
-    uint32_t pad;
-    uint32_t heavy[kMaxHeavy];
+    uint32_t kmin;        // min key of both sides
+    uint32_t shift;       // key bits dropped
 };
 
 // One window of a batched launch (device descriptor, built on the host).
@@ -105,6 +116,7 @@ struct BatchArgs {
     uint32_t grab;               // join items taken per queue access
     uint32_t idcap;              // run-id table entries of a PartView
     const RmiParams *rmi;
+    uint64_t *heavy;             // learned + grouped: 4 u64 per (window, heavy key)
 };
 
 // Launch configuration of the batched path, chosen by sjb_batch_autotune.
@@ -123,6 +135,7 @@ struct WindowPlan {
     uint32_t flags;
     uint64_t nL, nR;
     int ncL, ncR;  // chunks per side
+    bool peel;     // learned + grouped: heavy keys aggregated apart
 };
 
 }  // namespace sjb
@@ -149,6 +162,14 @@ struct sjb_ctx {
     sjb::PartParams *d_pparams = nullptr;
     sjb::RmiParams *d_rmi = nullptr;
     bool rmi_valid = false;
+    uint32_t rmi_nheavy = 0;          // host copy of d_rmi->n_heavy
+    // learned-model fit buffers (rmi.hip), allocated on the first fit
+    uint32_t *d_fit_keys = nullptr;   // sorted key sample
+    float *d_fit_x = nullptr, *d_fit_t = nullptr;
+    float *d_mlp = nullptr;           // params, grads, Adam m, Adam v (4 x kMlpParams)
+    double *d_fit_acc = nullptr;      // loss accumulator + leaf statistics
+    float *d_loss_hist = nullptr;
+    uint32_t loss_hist_cap = 0;
     uint64_t *d_cursor = nullptr;     // pair-output cursor
     uint64_t *d_heavy = nullptr;      // heavy-key aggregates
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -215,7 +236,7 @@ int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const 
                    sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
 int launch_smj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
                     sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
-int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out);
+int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy);
 
 // ---- launchers (nlj.hip) ----
 int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
@@ -225,7 +246,11 @@ int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_t
                       uint64_t nR, uint32_t flags, sjb_result_tuple_t *out, uint64_t out_cap,
                       sjb_window_result_t *d_out, uint64_t *n_masks);
 
+int launch_part_sizes(hipStream_t s, const uint32_t *offs, int logF, int ncL, int ncR, uint32_t *sizes);
+
 // ---- api.hip ----
+int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *p);
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                    const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
                    uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
@@ -238,11 +263,14 @@ int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const 
 void batch_plan_free(struct sjb_batch_plan *p);
 int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias);
 // batched kernels (partition.hip / join.hip)
-int launch_partition_batch(hipStream_t s, int kind, const BatchArgs &a, uint32_t n_chunks,
+int launch_partition_batch(hipStream_t s, int kind, bool peel, const BatchArgs &a, uint32_t n_chunks,
                            size_t dyn_bytes);
+int launch_heavy_finalize(hipStream_t s, sjb_window_result_t *res, uint32_t n, const uint64_t *heavy,
+                          uint32_t n_heavy);
 int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a, int blocks,
                       size_t dyn_bytes);
-int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead);
+int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead,
+                       uint64_t *heavy);
 size_t partition_lds_bytes_chunk(int logF, int chunk);
 size_t join_view_bytes(int alg, int ncL, int ncR);
 int join_idcap(int alg);

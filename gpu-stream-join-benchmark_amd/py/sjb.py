@@ -68,6 +68,36 @@ class BatchInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("tune", BatchTune)]
 
 
+class RmiConfig(ctypes.Structure):
+    _fields_ = [("root", ctypes.c_uint32), ("steps", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("heavy_min", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+class RmiReport(ctypes.Structure):
+    _fields_ = [("loss_init", ctypes.c_double), ("loss_final", ctypes.c_double),
+                ("fit_ms", ctypes.c_double), ("imbalance_learned", ctypes.c_double),
+                ("imbalance_radix", ctypes.c_double), ("n_heavy", ctypes.c_uint32),
+                ("leaves_used", ctypes.c_uint32), ("n_sample", ctypes.c_uint32),
+                ("fallback", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PartitionReport(ctypes.Structure):
+    _fields_ = [("fanout_log2", ctypes.c_uint32), ("kind", ctypes.c_uint32),
+                ("max_build", ctypes.c_uint64), ("max_probe", ctypes.c_uint64),
+                ("mean_build", ctypes.c_double), ("imbalance", ctypes.c_double),
+                ("fixup_frac", ctypes.c_double), ("heavy_left", ctypes.c_uint64),
+                ("heavy_right", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+RMI_LINEAR, RMI_MLP = 0, 1
+MLP_PARAMS = 1153
+
 # exported symbols and their signatures: (restype, argtypes)
 _u64, _u32, _i, _p, _d = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
 SIGNATURES = {
@@ -91,6 +121,11 @@ SIGNATURES = {
                                 ctypes.POINTER(_u32)]),
     "sjb_batch_get_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
     "sjb_batch_set_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
+    "sjb_rmi_fit": (_i, [_p, _p, _u64, ctypes.POINTER(RmiConfig), ctypes.POINTER(RmiReport)]),
+    "sjb_rmi_clear": (_i, [_p]),
+    "sjb_mlp_grad": (_i, [_p, _p, _p, _p, _u32, _p, ctypes.POINTER(_d)]),
+    "sjb_partition_report": (_i, [_p, _i, _u32, _p, _u64, _p, _u64, ctypes.POINTER(PartitionReport)]),
+    "sjb_generate_mix": (_i, [_p, _u64, _u64, _d, _d, _u64, _u64]),
     "sjb_eager_execute": (_i, [_p, _i, _i, _u32, _p, _u64, _i, _p, _u64, ctypes.POINTER(Stats)]),
     "sjb_eager_clear": (_i, [_p]),
     "sjb_eager_end_of_stream": (_i, [_p]),
@@ -137,6 +172,14 @@ def ptr(a):
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     return a.ctypes.data
+
+
+def generate_mix(n, window, zipf_frac=0.3, theta=1.2, seed=123, ts_base=0):
+    """Drifting 8-component Gaussian mixture over u32 keys mixed with Zipf keys."""
+    out = np.empty(n, dtype=TUPLE_DTYPE)
+    check(lib().sjb_generate_mix(ptr(out), n, window, zipf_frac, theta, seed, ts_base),
+          "sjb_generate_mix")
+    return out
 
 
 def generate(n, dist=DIST_UNIFORM, key_range=1 << 16, theta=0.0, seed=123, ts_base=0):
@@ -249,6 +292,32 @@ class Context:
                 setattr(t, k, v)
             check(lib().sjb_batch_set_tune(self.h, ctypes.byref(t)), "sjb_batch_set_tune")
         return t
+
+    def rmi_fit(self, train, n_train, root=RMI_MLP, steps=400, lr=0.01, seed=3, heavy_min=3):
+        cfg = RmiConfig(root, steps, lr, seed, heavy_min, 0)
+        rep = RmiReport()
+        check(lib().sjb_rmi_fit(self.h, ptr(train), n_train, ctypes.byref(cfg), ctypes.byref(rep)),
+              "sjb_rmi_fit")
+        return rep
+
+    def rmi_clear(self):
+        check(lib().sjb_rmi_clear(self.h), "sjb_rmi_clear")
+
+    def mlp_grad(self, params, x, t):
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(lib().sjb_mlp_grad(self.h, ptr(P), ptr(x), ptr(t), len(x), ptr(g), ctypes.byref(loss)),
+              "sjb_mlp_grad")
+        return loss.value, g
+
+    def partition_report(self, alg, flags, left, n_left, right, n_right):
+        rep = PartitionReport()
+        check(lib().sjb_partition_report(self.h, alg, flags, ptr(left), n_left, ptr(right), n_right,
+                                         ctypes.byref(rep)), "sjb_partition_report")
+        return rep
 
     def eager_execute(self, alg, method, flags, batch, is_left, out=None, out_cap=0):
         st = Stats()

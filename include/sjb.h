@@ -238,6 +238,66 @@ int sjb_batch_autotune(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
 int sjb_batch_get_tune(sjb_ctx_t *ctx, sjb_batch_tune_t *tune);
 int sjb_batch_set_tune(sjb_ctx_t *ctx, const sjb_batch_tune_t *tune);
 
+/* ---- learned partitioner ------------------------------------------------ */
+/* A two-stage RMI over the key CDF (root: linear or a 1-32-32-1 ReLU MLP
+ * trained on the device by Adam; 1024 linear leaves), fitted on a sample of
+ * earlier windows' tuples (d_train, device memory).  Joins called with
+ * SJB_F_LEARNED then partition by floor(F * cdf(key)) (equi-depth, also for
+ * SMJ), and in grouped mode the model's heavy keys (up to 32 most frequent
+ * sample keys occurring at least heavy_min times) are aggregated apart
+ * instead of partitioned.  The fit compares the learned partition of the
+ * sample with MSD radix on the key bits; when the learned one is more than
+ * 1.25x as imbalanced the model is not used (report->fallback = 1) and
+ * SJB_F_LEARNED joins take the plain radix / hash path.  Results never
+ * depend on the model: only partition balance does.  No reference
+ * counterpart (the reference partitions by key % D only).                 */
+#define SJB_RMI_LINEAR 0
+#define SJB_RMI_MLP 1
+typedef struct sjb_rmi_config {
+    uint32_t root;       /* SJB_RMI_LINEAR or SJB_RMI_MLP                   */
+    uint32_t steps;      /* MLP: full-batch Adam steps                      */
+    float lr;            /* MLP: peak learning rate (cosine decay to 10%)   */
+    uint32_t seed;       /* MLP: initial weights                            */
+    uint32_t heavy_min;  /* min sample count of a heavy key (0 = none)      */
+    uint32_t pad;
+} sjb_rmi_config_t;
+typedef struct sjb_rmi_report {
+    double loss_init;         /* MLP: mean squared CDF error before training */
+    double loss_final;        /* ... and after the last step                 */
+    double fit_ms;            /* device time of the whole fit                */
+    double imbalance_learned; /* max/mean partition size of the sample      */
+    double imbalance_radix;   /* same for MSD radix on the key bits          */
+    uint32_t n_heavy, leaves_used, n_sample, fallback;
+} sjb_rmi_report_t;
+int sjb_rmi_fit(sjb_ctx_t *ctx, const sjb_tuple_t *d_train, uint64_t n_train,
+                const sjb_rmi_config_t *cfg, sjb_rmi_report_t *report);
+/* Forgets the model (SJB_F_LEARNED then takes the radix / hash path).      */
+int sjb_rmi_clear(sjb_ctx_t *ctx);
+/* One forward/backward pass of the MLP root on host arrays: params
+ * (1153 floats: W1[32], b1[32], W2[32][32] row = output unit, b2[32],
+ * w3[32], b3), samples x[n], targets t[n] (n <= 65536); writes the gradient
+ * of mean((y - t)^2) and the loss.  Exposed for gradient checks.          */
+int sjb_mlp_grad(sjb_ctx_t *ctx, const float *params, const float *x, const float *t, uint32_t n,
+                 float *grad, double *loss);
+
+/* Partition statistics of one window (the learned path's judge): runs the
+ * partition pass of sjb_lazy_join(algorithm, flags) and reports per-side
+ * partition sizes.  fixup_frac = share of build tuples in partitions larger
+ * than one LDS tile (they take the multi-tile path).                       */
+typedef struct sjb_partition_report {
+    uint32_t fanout_log2;
+    uint32_t kind;            /* 0 hash, 1 range (radix), 2 learned          */
+    uint64_t max_build, max_probe;
+    double mean_build;
+    double imbalance;         /* max_build / mean_build                      */
+    double fixup_frac;
+    uint64_t heavy_left, heavy_right; /* tuples aggregated as heavy keys     */
+} sjb_partition_report_t;
+int sjb_partition_report(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
+                         const sjb_tuple_t *d_left, uint64_t n_left,
+                         const sjb_tuple_t *d_right, uint64_t n_right,
+                         sjb_partition_report_t *report);
+
 /* ---- eager (per-batch) join -------------------------------------------- */
 /* Joins one incoming batch against the opposite side's resident window state
  * and then appends it to its own side's state.  Replaces
@@ -274,6 +334,14 @@ uint64_t sjb_pk_launches(sjb_ctx_t *ctx);
 #define SJB_DIST_ZIPF 1
 int sjb_generate(sjb_tuple_t *out, uint64_t n, int dist, uint64_t key_range, double theta,
                  uint64_t seed, uint64_t ts_base);
+/* Drifting mixture over u32 keys for window `window`: with probability
+ * zipf_frac a Zipf(theta) key over [0, 2^32) (ranks scrambled as above),
+ * otherwise a draw of one of 8 Gaussian components: weight of component j
+ * proportional to 1 + (5j mod 4), mean (0.06 + 0.12 j) * 2^32 shifted by
+ * 0.01 * 2^32 every 64 windows (mod 2^32), sigma (0.004 + 0.002 (j mod 4)) *
+ * 2^32.  val and ts as sjb_generate.                                       */
+int sjb_generate_mix(sjb_tuple_t *out, uint64_t n, uint64_t window, double zipf_frac, double theta,
+                     uint64_t seed, uint64_t ts_base);
 
 /* ---- checksum primitives (host reference of the device formulas) ------ */
 uint64_t sjb_mix64(uint64_t z);

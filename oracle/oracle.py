@@ -56,6 +56,8 @@ def lib():
             "or_eager_execute": (_i, [_p, _p, _u64, _i, _i, _R, _p, _u64]),
             "or_sliding": (_i, [_p, _p, _u64, _u64, _u64, _p]),
             "or_bench_lazy_hj": (ctypes.c_double, [_p, _p, _u64, _u64, _u64, _i, _R]),
+            "or_mlp_grad": (ctypes.c_double, [_p, _p, _p, _u32, _p]),
+            "or_mlp_loss": (ctypes.c_double, [_p, _p, _p, _u32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -142,3 +144,23 @@ def bench_lazy_hj(R, S, W, D, threads):
 def sort_pairs(p):
     """Canonical order of a pair list (by every field)."""
     return np.sort(p, order=["lkey", "lval", "rval", "lts", "rts", "rkey"])
+
+
+MLP_PARAMS = 1153
+
+
+def mlp_grad(P, x, t):
+    """float64 loss and gradient of the learned partitioner's MLP root."""
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.zeros(MLP_PARAMS, dtype=np.float64)
+    loss = lib().or_mlp_grad(P.ctypes.data, x.ctypes.data, t.ctypes.data, len(x), g.ctypes.data)
+    return loss, g
+
+
+def mlp_loss(P, x, t):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    return lib().or_mlp_loss(P.ctypes.data, x.ctypes.data, t.ctypes.data, len(x))

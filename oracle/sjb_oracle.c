@@ -368,3 +368,62 @@ double or_bench_lazy_hj(const or_tuple *R, const or_tuple *S, uint64_t n, uint64
     free(occ); free(prefix); free(ins); free(store);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+
+/* ---- learned partitioner root (float64) ---------------------------------- */
+#define OR_H 32
+#define OR_W1 0
+#define OR_B1 32
+#define OR_W2 64
+#define OR_B2 (64 + OR_H * OR_H)
+#define OR_W3 (OR_B2 + OR_H)
+#define OR_B3 (OR_W3 + OR_H)
+
+static double mlp_forward(const double *P, double x, double *h1, double *h2) {
+    double y = P[OR_B3];
+    for (int j = 0; j < OR_H; j++) {
+        const double z = P[OR_W1 + j] * x + P[OR_B1 + j];
+        h1[j] = z > 0 ? z : 0;
+    }
+    for (int i = 0; i < OR_H; i++) {
+        double z = P[OR_B2 + i];
+        for (int j = 0; j < OR_H; j++) z += P[OR_W2 + i * OR_H + j] * h1[j];
+        h2[i] = z > 0 ? z : 0;
+        y += P[OR_W3 + i] * h2[i];
+    }
+    return y;
+}
+
+double or_mlp_loss(const double *P, const double *x, const double *t, uint32_t n) {
+    double h1[OR_H], h2[OR_H], loss = 0;
+    for (uint32_t s = 0; s < n; s++) {
+        const double e = mlp_forward(P, x[s], h1, h2) - t[s];
+        loss += e * e;
+    }
+    return loss / n;
+}
+
+double or_mlp_grad(const double *P, const double *x, const double *t, uint32_t n, double *grad) {
+    double h1[OR_H], h2[OR_H], dz2[OR_H], loss = 0;
+    memset(grad, 0, sizeof(double) * (OR_B3 + 1));
+    for (uint32_t s = 0; s < n; s++) {
+        const double e = mlp_forward(P, x[s], h1, h2) - t[s];
+        loss += e * e;
+        const double dy = 2.0 * e / n;
+        grad[OR_B3] += dy;
+        for (int i = 0; i < OR_H; i++) {
+            grad[OR_W3 + i] += dy * h2[i];
+            dz2[i] = h2[i] > 0 ? dy * P[OR_W3 + i] : 0;
+            grad[OR_B2 + i] += dz2[i];
+            for (int j = 0; j < OR_H; j++) grad[OR_W2 + i * OR_H + j] += dz2[i] * h1[j];
+        }
+        for (int j = 0; j < OR_H; j++) {
+            if (h1[j] <= 0) continue;
+            double dh = 0;
+            for (int i = 0; i < OR_H; i++) dh += dz2[i] * P[OR_W2 + i * OR_H + j];
+            grad[OR_W1 + j] += dh * x[s];
+            grad[OR_B1 + j] += dh;
+        }
+    }
+    return loss / n;
+}

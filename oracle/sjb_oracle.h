@@ -82,6 +82,16 @@ int or_sliding(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uin
 double or_bench_lazy_hj(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uint64_t D,
                         int threads, or_result *total);
 
+/* Learned partitioner root, float64: the 1-32-32-1 ReLU MLP
+ *   y = w3 . relu(W2 relu(W1 x + b1) + b2) + b3,  loss = mean (y - t)^2
+ * with parameters P[1153] = W1[32], b1[32], W2[32][32] (row i = output unit
+ * i), b2[32], w3[32], b3.  Writes d loss / d P into grad and returns the
+ * loss (analytic back-propagation; relu'(0) = 0).  No reference
+ * counterpart: pinned by central finite differences in tests.           */
+double or_mlp_grad(const double *P, const double *x, const double *t, uint32_t n, double *grad);
+/* loss only (for finite differences) */
+double or_mlp_loss(const double *P, const double *x, const double *t, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,142 @@
+"""GPU tests of the learned partitioner (two-stage RMI with a 1-32-32-1 MLP
+root trained on the device).
+
+* The device forward/backward kernel matches the float64 oracle gradient
+  (itself pinned by finite differences in test_oracle.py) within a relative
+  tolerance of 2e-3 of the gradient's max-norm (float32 accumulation with
+  atomics over 4096 samples).
+* Joins partitioned by the model are bit-exact (count, C1, C2 / grouped
+  count and C1 with peeled heavy keys) against the oracle: the model only
+  moves balance, never results.
+* What training gains: on the next window of the drifting mixture, the
+  trained MLP root gives lower partition imbalance and fewer fix-ups
+  (tuples in partitions larger than one LDS tile) than the linear-root RMI
+  and than radix (range) partitioning on the key bits.
+"""
+import numpy as np
+import pytest
+
+from conftest import to_dev
+
+pytestmark = pytest.mark.gpu
+
+HJ, SMJ = 2, 3
+F_C2, F_GROUPED, F_LEARNED = 1, 2, 4
+
+
+def training_windows(sjbmod, n, first, count, seed=40):
+    return np.concatenate([sjbmod.generate_mix(n, w, seed=seed + w) for w in range(first, first + count)])
+
+
+def test_mlp_grad_matches_oracle(ctx, orc):
+    rng = np.random.default_rng(11)
+    P = rng.normal(0, 0.4, 1153).astype(np.float32)
+    x = rng.uniform(0, 1, 4096).astype(np.float32)
+    t = np.sort(rng.uniform(0, 1, 4096)).astype(np.float32)
+    loss_gpu, g_gpu = ctx.mlp_grad(P, x, t)
+    loss_cpu, g_cpu = orc.mlp_grad(P.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    scale = np.abs(g_cpu).max()
+    assert scale > 0
+    assert np.abs(g_gpu - g_cpu).max() <= 2e-3 * scale
+    assert abs(loss_gpu - loss_cpu) <= 1e-5 * max(loss_cpu, 1e-12)
+
+
+def test_mlp_grad_tiles_and_tail(ctx, orc):
+    # n not a multiple of the 128-sample tile: the tail samples must count once
+    rng = np.random.default_rng(12)
+    P = rng.normal(0, 0.4, 1153).astype(np.float32)
+    for n in (1, 127, 129, 1000):
+        x = rng.uniform(0, 1, n).astype(np.float32)
+        t = rng.uniform(0, 1, n).astype(np.float32)
+        _, g_gpu = ctx.mlp_grad(P, x, t)
+        _, g_cpu = orc.mlp_grad(P.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+        assert np.abs(g_gpu - g_cpu).max() <= 2e-3 * max(np.abs(g_cpu).max(), 1e-30), n
+
+
+@pytest.fixture(scope="module")
+def fitted(sjbmod, torch_mod):
+    """A ctx with an MLP-root model fitted on windows 0..3 (2^18 tuples each)."""
+    train = training_windows(sjbmod, 1 << 18, 0, 4)
+    d_train = to_dev(torch_mod, train)
+    c = sjbmod.Context(max_window=1 << 22, distinct_keys=1 << 16)
+    rep = c.rmi_fit(d_train, len(train), root=sjbmod.RMI_MLP, steps=400, lr=0.01)
+    yield c, rep, d_train, len(train)
+    c.close()
+
+
+def test_fit_report(fitted):
+    _, rep, _, _ = fitted
+    assert rep.fallback == 0
+    assert 1 <= rep.n_heavy <= 32
+    assert rep.loss_final < rep.loss_init / 10
+    assert rep.imbalance_learned < rep.imbalance_radix
+    assert rep.leaves_used > 100
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_learned_grouped_exact(fitted, torch_mod, orc, sjbmod, alg):
+    c, _, _, _ = fitted
+    n = 1 << 18
+    R = sjbmod.generate_mix(n, 4, seed=901)
+    S = sjbmod.generate_mix(n, 4, seed=902)
+    o, _ = orc.lazy_smj(R, S)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    st = c.lazy_join(alg, 0, F_GROUPED | F_LEARNED, dR, n, dS, n)
+    assert (st.result_count, st.checksum1) == (o.count, o.c1)
+    rep = c.partition_report(alg, F_GROUPED | F_LEARNED, dR, n, dS, n)
+    assert rep.kind == 2 and rep.heavy_left > 0 and rep.heavy_right > 0
+    # plain path gives the same result
+    st2 = c.lazy_join(alg, 0, F_GROUPED, dR, n, dS, n)
+    assert (st2.result_count, st2.checksum1) == (o.count, o.c1)
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_learned_pairwise_exact(fitted, torch_mod, orc, sjbmod, alg):
+    # C2 enumerates pairs: heavy keys are partitioned, not peeled
+    c, _, _, _ = fitted
+    n = 1 << 15
+    R = sjbmod.generate_mix(n, 4, seed=911)
+    S = sjbmod.generate_mix(n, 4, seed=912)
+    o, _ = orc.lazy_hj(R, S, 1 << 16)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    st = c.lazy_join(alg, 0, F_C2 | F_LEARNED, dR, n, dS, n)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+
+
+def test_learned_batch_equals_single(fitted, torch_mod, sjbmod):
+    c, _, _, _ = fitted
+    sizes = np.array([1 << 14, 300, 1 << 16, 7, 5000, 1 << 12], dtype=np.uint64)
+    lo = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    R = sjbmod.generate_mix(int(lo[-1]), 5, seed=921)
+    S = sjbmod.generate_mix(int(lo[-1]), 5, seed=922)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    for alg in (HJ, SMJ):
+        res = c.lazy_join_batch(alg, F_GROUPED | F_LEARNED, dR, lo, dS, lo)
+        for w in range(len(sizes)):
+            a, b = int(lo[w]), int(lo[w + 1])
+            st = c.lazy_join(alg, 0, F_GROUPED, dR[a * 16:], b - a, dS[a * 16:], b - a)
+            assert (int(res[w]["count"]), int(res[w]["checksum1"])) == (st.result_count, st.checksum1)
+
+
+def test_trained_network_gains(sjbmod, torch_mod):
+    # the next window after the training windows: MLP root vs linear root vs radix
+    train = training_windows(sjbmod, 1 << 20, 0, 4, seed=70)
+    d_train = to_dev(torch_mod, train)
+    n = 1 << 22
+    R = sjbmod.generate_mix(n, 4, seed=971)
+    S = sjbmod.generate_mix(n, 4, seed=972)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    with sjbmod.Context(max_window=n, distinct_keys=1 << 16) as c:
+        radix = c.partition_report(SMJ, F_GROUPED, dR, n, dS, n)
+        lin_fit = c.rmi_fit(d_train, len(train), root=sjbmod.RMI_LINEAR)
+        lin = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
+        mlp_fit = c.rmi_fit(d_train, len(train), root=sjbmod.RMI_MLP, steps=400, lr=0.01)
+        mlp = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
+        c.rmi_clear()
+        plain = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
+    print("radix", radix.as_dict(), "\nlinear", lin.as_dict(), lin_fit.as_dict(),
+          "\nmlp", mlp.as_dict(), mlp_fit.as_dict(), flush=True)
+    assert radix.kind == 1 and lin.kind == 2 and mlp.kind == 2 and plain.kind == 1
+    assert mlp_fit.loss_final < mlp_fit.loss_init / 10
+    assert mlp.imbalance < lin.imbalance < radix.imbalance
+    assert mlp.fixup_frac < lin.fixup_frac < radix.fixup_frac

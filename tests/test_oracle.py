@@ -147,3 +147,40 @@ def test_cpu_baseline_runs(orc):
         r, _ = orc.lazy_hj(R[w:w + W], S[w:w + W], 1 << 12)
         exp = [(a + b) % 2**64 for a, b in zip(exp, r.triple())]
     assert tot.triple() == tuple(exp)
+
+
+# ---- learned partitioner root: the float64 MLP gradient ------------------
+def _mlp_case(seed, n):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(0, 0.5, 1153)
+    x = rng.uniform(0, 1, n)
+    t = np.sort(rng.uniform(0, 1, n))
+    return P, x, t
+
+
+def test_mlp_oracle_gradient_finite_differences(orc):
+    # central differences pin the analytic back-propagation of the oracle
+    P, x, t = _mlp_case(1, 50)
+    loss, g = orc.mlp_grad(P, x, t)
+    assert abs(loss - orc.mlp_loss(P, x, t)) < 1e-15
+    rng = np.random.default_rng(2)
+    idx = np.concatenate([np.arange(0, 64), rng.choice(np.arange(64, 1153), 120, replace=False)])
+    eps = 1e-6
+    for i in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += eps
+        Pm[i] -= eps
+        fd = (orc.mlp_loss(Pp, x, t) - orc.mlp_loss(Pm, x, t)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_mlp_oracle_special_cases(orc):
+    # all weights zero except b3: y = b3, d loss / d b3 = 2 mean(b3 - t)
+    P = np.zeros(1153)
+    P[1152] = 0.25
+    x = np.linspace(0, 1, 11)
+    t = np.linspace(0, 1, 11)
+    loss, g = orc.mlp_grad(P, x, t)
+    assert abs(loss - np.mean((0.25 - t) ** 2)) < 1e-15
+    assert abs(g[1152] - 2 * np.mean(0.25 - t)) < 1e-15
+    assert np.all(g[:1152] == 0)  # every hidden unit is dead (relu(0) = 0, relu'(0) = 0)
