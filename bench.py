@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 128 MiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", default="window", choices=["window", "batch"],
+    ap.add_argument("--workload", default="window", choices=["window", "batch", "learned"],
                     help="window: 2^22-tuple tumbling windows (default); batch: 4096 lognormal "
                          "windows per launch, Zipf keys over [0, 2^28)")
     ap.add_argument("--theta", type=float, default=0.0, help="batch: Zipf theta (0 = uniform)")
@@ -54,6 +54,109 @@ def lognormal_sizes(n, seed):
     rng = np.random.default_rng(seed)
     sz = np.exp(rng.normal(np.log(2.0 ** 14), 1.0, size=n))
     return np.clip(np.round(sz), 2 ** 8, 2 ** 20).astype(np.uint64)
+
+
+def time_plan(ctx, plan, steps, warmup, torch):
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for _ in range(warmup):
+        plan.launch()
+    ctx.sync()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for s in range(steps):
+        plan.launch()
+        ev[s + 1].record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    return [ev[s].elapsed_time(ev[s + 1]) for s in range(steps)]
+
+
+def learned_case(a, first, nw, torch, sjb, np):
+    """Fit on windows first-4..first-1, time windows first..first+nw-1 for
+    HJ and SMJ, radix (plain) vs learned partitioning, both batched."""
+    W = 1 << a.log_window
+    train = np.concatenate([sjb.generate_mix(W, w, seed=1000 + w) for w in range(first - 4, first)])
+    R = np.concatenate([sjb.generate_mix(W, w, seed=1000 + w) for w in range(first, first + nw)])
+    S = np.concatenate([sjb.generate_mix(W, w, seed=5000 + w) for w in range(first, first + nw)])
+    d_train = torch.from_numpy(train.view(np.uint8)).to("cuda")
+    dR = torch.from_numpy(R.view(np.uint8)).to("cuda")
+    dS = torch.from_numpy(S.view(np.uint8)).to("cuda")
+    off = np.arange(nw + 1, dtype=np.uint64) * np.uint64(W)
+    ctx = sjb.Context(device=0, max_window=W, distinct_keys=1 << 20)
+    res = {}
+    for name in ("HJ", "SMJ"):
+        alg = sjb.ALG[name]
+        out = {}
+        for mode in ("radix", "learned"):
+            f = sjb.F_GROUPED | (sjb.F_LEARNED if mode == "learned" else 0)
+            if mode == "learned":
+                fit = ctx.rmi_fit(d_train, len(train), root=sjb.RMI_MLP, steps=400, lr=0.01)
+                out["fit"] = {k: fit.as_dict()[k] for k in ("fit_ms", "loss_init", "loss_final", "n_heavy",
+                                                          "fallback")}
+            else:
+                ctx.rmi_clear()
+            rep = ctx.partition_report(alg, f, dR, W, dS, W)
+            best, tried = ctx.batch_autotune(alg, f, dR, off, dS, off, reps=2)
+            d_res = torch.zeros(nw * 32, dtype=torch.uint8, device="cuda")
+            plan = ctx.batch_plan(alg, f, dR, off, dS, off, d_res)
+            ms = time_plan(ctx, plan, a.steps, a.warmup, torch)
+            r = d_res.cpu().numpy().view(sjb.WRES_DTYPE).copy()
+            plan.destroy()
+            med = float(np.median(ms))
+            out[mode] = {"ms_per_step": med, "tuples_per_s": 2 * W * nw / (med / 1e3),
+                         "p99_ms": float(np.percentile(ms, 99)),
+                         "partition": {k: rep.as_dict()[k] for k in ("kind", "imbalance", "fixup_frac",
+                                                                    "max_build", "heavy_left")},
+                         "tune": best.as_dict(),
+                         "results": [(int(x["count"]), int(x["checksum1"])) for x in r]}
+        if out["radix"]["results"] != out["learned"]["results"]:
+            raise SystemExit(f"{name}: learned and radix results differ")
+        out["speedup"] = out["radix"]["ms_per_step"] / out["learned"]["ms_per_step"]
+        for m in ("radix", "learned"):
+            out[m].pop("results")
+        res[name] = out
+    ctx.close()
+    return res
+
+
+def bench_learned(a):
+    """Learned vs plain radix partitioning on drifting keys: 2^22-tuple windows
+    per side, keys 70% 8-component Gaussian mixture over u32 (means shift 1% of
+    the range every 64 windows) + 30% Zipf(1.2).  The RMI (2x32 MLP root, 1024
+    leaves, up to 32 heavy keys) is fitted on the 4 windows before the timed
+    batch of 8 windows.  Headline case: windows 68..75 (same drift epoch as the
+    training windows 64..67); also reported: windows 64..71 right after a drift
+    step, trained on 60..63.  Count + C1 (grouped).  A step is one replay of the
+    batch plan over the 8 windows."""
+    import numpy as np
+    import torch
+    import sjb
+
+    W = 1 << a.log_window
+    nw = 8
+    t0 = time.time()
+    typical = learned_case(a, 68, nw, torch, sjb, np)
+    boundary = learned_case(a, 64, nw, torch, sjb, np)
+    print(f"[bench] learned: {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    hj = typical["HJ"]["learned"]
+    alg_bytes = 16 * 2 * W * nw
+    line = {
+        "metric": "joined input tuples/s, learned+batched vs plain radix partitioning (drifting GMM + Zipf 1.2)",
+        "value": hj["tuples_per_s"], "unit": "tuples/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": hj["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 keys / u64 sums",
+        "data": "synthetic (drifting 8-component GMM over u32 + 30% Zipf(1.2), splitmix64 generator)",
+        "config": {"workload": f"learned-partition HJ/SMJ, {nw} windows of 2^{a.log_window}/side per step, "
+                               "RMI (2x32 MLP root, 1024 leaves) fitted on the 4 previous windows",
+                   "checksums": "count+C1 (grouped)", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "batch pipeline (learned HJ)",
+                     "achieved": alg_bytes / (hj["ms_per_step"] / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": alg_bytes / (hj["ms_per_step"] / 1e3) / HBM_PEAK, "traffic": None},
+        "speedup_vs_radix": {k: v["speedup"] for k, v in typical.items()},
+        "speedup_vs_radix_after_drift_step": {k: v["speedup"] for k, v in boundary.items()},
+        "detail": {"same_epoch": typical, "after_drift_step": boundary},
+    }
+    print(json.dumps(line))
 
 
 def bench_batch(a):
@@ -147,6 +250,8 @@ def main():
         sys.exit(2)
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "learned":
+        return bench_learned(a)
     import numpy as np
     import torch
     import sjb

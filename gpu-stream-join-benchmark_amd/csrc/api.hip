@@ -102,11 +102,10 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.nL = nL;
     p.nR = nR;
     p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2));
-    p.kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
-             : alg == SJB_ALG_SMJ                     ? PART_RANGE
-                                                      : PART_HASH;
+    const bool learned = (flags & SJB_F_LEARNED) && c->rmi_valid;
+    p.kind = part_kind(alg, learned);
     p.payload = write ? PAY_IDX : PAY_VAL;
-    p.peel = p.kind == PART_LEARNED && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
+    p.peel = learned && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
     p.ncL = int((nL + kChunk - 1) / kChunk);
     p.ncR = int((nR + kChunk - 1) / kChunk);
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;

@@ -282,10 +282,9 @@ static int plan_init(sjb_ctx *c, sjb_batch_plan *p, int alg, uint32_t flags, uin
     p->c = c;
     p->alg = alg;
     p->flags = flags;
-    p->kind = (flags & SJB_F_LEARNED) && c->rmi_valid ? PART_LEARNED
-              : alg == SJB_ALG_SMJ                     ? PART_RANGE
-                                                       : PART_HASH;
-    p->peel = p->kind == PART_LEARNED && (flags & SJB_F_GROUPED) && c->rmi_nheavy > 0;
+    const bool learned = (flags & SJB_F_LEARNED) && c->rmi_valid;
+    p->kind = part_kind(alg, learned);
+    p->peel = learned && (flags & SJB_F_GROUPED) && c->rmi_nheavy > 0;
     p->n_heavy = p->peel ? c->rmi_nheavy : 0;
     p->tune = c->tune;
     p->n_win = n;

@@ -429,26 +429,31 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
 // C1 factorises per key: sum over pairs of A(r)*B(s) = sum over probe
 // records s of B(s) * (sum of A over the build records with s's key).  The
 // build side of a partition is aggregated into an LDS open-addressing table
-// of distinct keys (key, count, sum of A); a probe record costs one lookup
-// whatever the key's multiplicity, so the work is O(n) at any skew.  The
-// table is filled in rounds of at most kJoinThreads records; before a round
-// starts with more than kAggHigh distinct keys in the table, every probe
-// record is looked up against it and it is cleared, so the table never
-// exceeds 3/4 load.  Key 0xFFFFFFFF (the empty marker) has its own slot.
+// of distinct keys: count, the val of the key's first record, and the sum
+// of A over its other records — A of the first record is only computed if a
+// probe matches, so unique build keys cost no hash at build time.  A probe
+// record costs one lookup whatever the key's multiplicity, so the work is
+// O(n) at any skew.  The table is filled in rounds of at most kJoinThreads
+// records; before a round starts with more than kAggHigh distinct keys in
+// the table, every probe record is looked up against it and it is cleared,
+// so the table never exceeds 3/4 load.  Key 0xFFFFFFFF (the empty marker)
+// has its own slot.
 constexpr int kAggLog = 11;
 constexpr int kAggSlots = 1 << kAggLog;
 constexpr uint32_t kAggHigh = kAggSlots / 2;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr int kAggProbe = 2048;  // probe records staged in LDS per pass
 static_assert(kAggHigh + kJoinThreads <= (kAggSlots * 3) / 4, "aggregation table load");
 
 struct AggLds {
     uint32_t key[kAggSlots];
     uint32_t cnt[kAggSlots];
-    uint64_t suma[kAggSlots];
-    uint64_t ff_suma;  // key 0xFFFFFFFF
-    uint32_t ff_cnt;
+    uint32_t fval[kAggSlots];   // val of the record that made the count 0 -> n
+    uint64_t suma[kAggSlots];   // sum of A over the other records
+    uint64_t ff_suma;           // key 0xFFFFFFFF
+    uint32_t ff_cnt, ff_fval;
     uint32_t distinct;
-    Rec probe[kHjCap];
+    Rec probe[kAggProbe];
 };
 
 __device__ __forceinline__ uint32_t agg_slot(uint32_t k, int bshift) {
@@ -469,12 +474,9 @@ __device__ __forceinline__ void agg_clear(AggLds &t) {
     }
 }
 
-__device__ __forceinline__ void agg_add(AggLds &t, uint32_t k, uint32_t cnt, uint64_t sa, int bshift) {
-    if (k == kEmptyKey) {
-        atomicAdd(&t.ff_cnt, cnt);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&t.ff_suma), (unsigned long long)sa);
-        return;
-    }
+// Slot of key k (inserted if absent); kAggSlots for key 0xFFFFFFFF.
+__device__ __forceinline__ uint32_t agg_find_insert(AggLds &t, uint32_t k, int bshift) {
+    if (k == kEmptyKey) return kAggSlots;
     uint32_t s = agg_slot(k, bshift);
     for (;;) {
         uint32_t cur = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -482,38 +484,56 @@ __device__ __forceinline__ void agg_add(AggLds &t, uint32_t k, uint32_t cnt, uin
             cur = atomicCAS(&t.key[s], kEmptyKey, k);
             if (cur == kEmptyKey) {
                 atomicAdd(&t.distinct, 1u);
-                cur = k;
+                return s;
             }
         }
-        if (cur == k) break;
+        if (cur == k) return s;
         s = (s + 1) & (kAggSlots - 1);
     }
-    atomicAdd(&t.cnt[s], cnt);
-    atomicAdd(reinterpret_cast<unsigned long long *>(&t.suma[s]), (unsigned long long)sa);
 }
 
-// Inserts one record per lane (`valid` lanes).  The lanes sharing the first
-// valid lane's key are combined into one table update (the common case
-// under heavy skew, where one key dominates a partition).
+// Adds `n` records of key k to its slot: `v` is one of them (the one that
+// becomes the slot's first record if the count was 0) and `rest` the sum of
+// A over the other n-1.
+__device__ __forceinline__ void agg_add(AggLds &t, uint32_t k, uint32_t n, uint32_t v, uint64_t rest,
+                                        int bshift) {
+    const uint32_t s = agg_find_insert(t, k, bshift);
+    uint32_t *cnt = s == kAggSlots ? &t.ff_cnt : &t.cnt[s];
+    uint64_t *sum = s == kAggSlots ? &t.ff_suma : &t.suma[s];
+    const uint32_t old = atomicAdd(cnt, n);
+    if (old == 0)
+        (s == kAggSlots ? t.ff_fval : t.fval[s]) = v;
+    else
+        rest += hash_a(k, v);
+    if (rest) atomicAdd(reinterpret_cast<unsigned long long *>(sum), (unsigned long long)rest);
+}
+
+// Inserts one record per lane (`valid` lanes).  When several lanes of the
+// wave share the first valid lane's key (the common case under heavy skew,
+// where one key dominates a partition), they are combined into one update.
 __device__ __forceinline__ void agg_insert(AggLds &t, bool valid, Rec r, int bshift) {
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t a = valid ? hash_a(r.x, r.y) : 0ull;
     const int lead = __ffsll((unsigned long long)vmask) - 1;
     const uint32_t k0 = __shfl(r.x, lead, kWave);
     const bool same = valid && r.x == k0;
     const uint64_t smask = __ballot(same);
-    const uint64_t sa = wave_sum(same ? a : 0ull);
-    if (lane == lead) agg_add(t, k0, uint32_t(__popcll(smask)), sa, bshift);
-    if (valid && !same) agg_add(t, r.x, 1u, a, bshift);
+    if (__popcll(smask) > 1) {  // wave-uniform
+        const uint64_t sa = wave_sum(same && lane != lead ? hash_a(r.x, r.y) : 0ull);
+        if (lane == lead) agg_add(t, k0, uint32_t(__popcll(smask)), r.y, sa, bshift);
+        if (valid && !same) agg_add(t, r.x, 1u, r.y, 0ull, bshift);
+    } else if (valid) {
+        agg_add(t, r.x, 1u, r.y, 0ull, bshift);
+    }
 }
 
 __device__ __forceinline__ void agg_probe(const AggLds &t, Rec s, int bshift, Acc &acc) {
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, fv = 0;
     uint64_t sa = 0;
     if (s.x == kEmptyKey) {
         cnt = t.ff_cnt;
+        fv = t.ff_fval;
         sa = t.ff_suma;
     } else {
         uint32_t sl = agg_slot(s.x, bshift);
@@ -521,6 +541,7 @@ __device__ __forceinline__ void agg_probe(const AggLds &t, Rec s, int bshift, Ac
             const uint32_t cur = t.key[sl];
             if (cur == s.x) {
                 cnt = t.cnt[sl];
+                fv = t.fval[sl];
                 sa = t.suma[sl];
                 break;
             }
@@ -530,7 +551,7 @@ __device__ __forceinline__ void agg_probe(const AggLds &t, Rec s, int bshift, Ac
     }
     if (cnt) {
         acc.count += cnt;
-        acc.c1 += sa * hash_b(s.x, s.y);
+        acc.c1 += (sa + hash_a(s.x, fv)) * hash_b(s.x, s.y);
     }
 }
 
@@ -539,15 +560,15 @@ __device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &
     const uint32_t nb = v.n[0], np = v.n[1];
     if (!nb || !np) return;
     const Rec *__restrict__ part = io.part;
-    const uint32_t pn0 = min<uint32_t>(np, kHjCap);
+    const uint32_t pn0 = min<uint32_t>(np, kAggProbe);
     agg_clear(t);
     gather(part, v, 1, 0, pn0, [&](uint32_t i, Rec r) { t.probe[i] = r; });
     __syncthreads();
     uint32_t resident = 0;
     // probe every probe record against the table
     auto probe_all = [&]() {
-        for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
-            const uint32_t un = min<uint32_t>(kHjCap, np - u0);
+        for (uint32_t u0 = 0; u0 < np; u0 += kAggProbe) {
+            const uint32_t un = min<uint32_t>(kAggProbe, np - u0);
             if (u0 != resident) {
                 __syncthreads();
                 gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { t.probe[i] = r; });

@@ -460,7 +460,9 @@ int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, cons
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
     if (uint64_t((1u << p.logF) + 1) * uint64_t(p.ncL + p.ncR) > c->offs_cap) return SJB_E_CAPACITY;
     switch (p.kind) {
-        case PART_HASH: return partition_kind<PART_HASH, false>(c, p, L, R);
+        case PART_HASH:
+            return p.peel ? partition_kind<PART_HASH, true>(c, p, L, R)
+                          : partition_kind<PART_HASH, false>(c, p, L, R);
         case PART_RANGE: return partition_kind<PART_RANGE, false>(c, p, L, R);
         case PART_LEARNED:
             return p.peel ? partition_kind<PART_LEARNED, true>(c, p, L, R)
@@ -483,6 +485,9 @@ int launch_partition_batch(hipStream_t s, int kind, bool peel, const BatchArgs &
     if (!big && a.chunk != 4096) return SJB_E_INVALID;
     switch (kind) {
         case PART_HASH:
+            if (peel)
+                return big ? partition_batch_kind<PART_HASH, kChunk, true>(s, a, n_chunks, dyn)
+                           : partition_batch_kind<PART_HASH, 4096, true>(s, a, n_chunks, dyn);
             return big ? partition_batch_kind<PART_HASH, kChunk, false>(s, a, n_chunks, dyn)
                        : partition_batch_kind<PART_HASH, 4096, false>(s, a, n_chunks, dyn);
         case PART_RANGE:
@@ -548,6 +553,9 @@ int partition_setup(sjb_ctx *c) {
     const void *fns[] = {
         (const void *)k_partition<PART_HASH, PAY_VAL, false>,
         (const void *)k_partition<PART_HASH, PAY_IDX, false>,
+        (const void *)k_partition<PART_HASH, PAY_VAL, true>,
+        (const void *)k_partition_batch<PART_HASH, kChunk, true>,
+        (const void *)k_partition_batch<PART_HASH, 4096, true>,
         (const void *)k_partition<PART_RANGE, PAY_VAL, false>,
         (const void *)k_partition<PART_RANGE, PAY_IDX, false>,
         (const void *)k_partition<PART_LEARNED, PAY_VAL, false>,

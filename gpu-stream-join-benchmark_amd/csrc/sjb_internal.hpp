@@ -53,6 +53,15 @@ constexpr int kHjLogNB = SJB_HJ_LOGNB;         // log2 LDS hash buckets (HJ)
 constexpr int kSmjCap = SJB_SMJ_CAP;           // elements per LDS sort tile (SMJ)
 
 enum PartKind : int { PART_HASH = 0, PART_RANGE = 1, PART_LEARNED = 2 };
+
+// Partition function of a window.  HJ hashes (equi-depth over distinct
+// keys already; a learned model only contributes its heavy keys, which
+// grouped joins peel); SMJ uses MSD radix on the key bits, or with a
+// learned model equi-depth CDF ranges.
+inline int part_kind(int alg, bool learned) {
+    if (alg != SJB_ALG_SMJ) return PART_HASH;
+    return learned ? PART_LEARNED : PART_RANGE;
+}
 enum Payload : int { PAY_VAL = 0, PAY_IDX = 1 };
 
 // Learned CDF model (two-stage RMI).  A key maps to x = (key - kmin) *

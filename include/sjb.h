@@ -241,11 +241,12 @@ int sjb_batch_set_tune(sjb_ctx_t *ctx, const sjb_batch_tune_t *tune);
 /* ---- learned partitioner ------------------------------------------------ */
 /* A two-stage RMI over the key CDF (root: linear or a 1-32-32-1 ReLU MLP
  * trained on the device by Adam; 1024 linear leaves), fitted on a sample of
- * earlier windows' tuples (d_train, device memory).  Joins called with
- * SJB_F_LEARNED then partition by floor(F * cdf(key)) (equi-depth, also for
- * SMJ), and in grouped mode the model's heavy keys (up to 32 most frequent
- * sample keys occurring at least heavy_min times) are aggregated apart
- * instead of partitioned.  The fit compares the learned partition of the
+ * earlier windows' tuples (d_train, device memory).  SMJ joins called with
+ * SJB_F_LEARNED partition by floor(F * cdf(key)) (equi-depth key ranges)
+ * instead of MSD radix; HJ keeps hash partitions, which are equi-depth over
+ * distinct keys already.  In grouped mode (both algorithms) the model's
+ * heavy keys (up to 32 most frequent sample keys occurring at least
+ * heavy_min times) are aggregated apart instead of partitioned.  The fit compares the learned partition of the
  * sample with MSD radix on the key bits; when the learned one is more than
  * 1.25x as imbalanced the model is not used (report->fallback = 1) and
  * SJB_F_LEARNED joins take the plain radix / hash path.  Results never
