@@ -124,9 +124,9 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
 // Enqueues one window (count/checksums or pair output) on the ctx stream;
 // the result lands in d_res.  write: 0 = count, 1 = materialise.
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
-                          const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
-                          uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
-                          bool timing) {
+                   const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
+                   uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
+                   const hipEvent_t *ev) {
     int rc;
     if (write) {
         rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
@@ -137,26 +137,26 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
         return launch_fill_u64(c->stream, reinterpret_cast<uint64_t *>(d_res), 0,
                                sizeof(sjb_window_result_t) / 8);
     }
-    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], c->stream));
     if (alg == SJB_ALG_NLJ) {
         *logF_used = 0;
-        if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], c->stream));
         rc = launch_nlj(c, L, nL, R, nR, flags, write, out, out_cap, wrap, d_res);
         if (rc) return rc;
-        if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+        if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
         return SJB_OK;
     }
     WindowPlan p;
     rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p);
     if (rc) return rc;
     *logF_used = p.logF;
-    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], c->stream));
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
     rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res, p.peel ? c->rmi_nheavy : 0);
     if (rc) return rc;
-    if (timing) SJB_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
     return SJB_OK;
 }
 
@@ -225,7 +225,8 @@ static int lazy_join_device(sjb_ctx *c, int alg, int method, uint32_t flags, con
         return res.count > out_cap ? SJB_E_CAPACITY : SJB_OK;
     }
     if (method == SJB_OUT_NOOUTPUT) {
-        rc = window_enqueue(c, alg, flags, L, nL, R, nR, 0, nullptr, 0, 0, c->d_result, &logF, timing);
+        rc = window_enqueue(c, alg, flags, L, nL, R, nR, 0, nullptr, 0, 0, c->d_result, &logF,
+                            timing ? c->ev : nullptr);
         if (rc) return rc;
         rc = fetch_result(c, &res);
         if (rc) return rc;
@@ -235,7 +236,7 @@ static int lazy_join_device(sjb_ctx *c, int alg, int method, uint32_t flags, con
         if (method == SJB_OUT_COUNTKERNEL) {
             // pass 1: exact count (GPU_HJ_CountKernel.cu:83-101 counts first)
             rc = window_enqueue(c, alg, flags & ~SJB_F_C2, L, nL, R, nR, 0, nullptr, 0, 0,
-                                c->d_result, &logF, false);
+                                c->d_result, &logF, nullptr);
             if (rc) return rc;
             rc = fetch_result(c, &res);
             if (rc) return rc;
@@ -249,7 +250,7 @@ static int lazy_join_device(sjb_ctx *c, int alg, int method, uint32_t flags, con
         if (method == SJB_OUT_ATOMIC && out_cap == 0) return SJB_E_INVALID;
         const int wrap = method == SJB_OUT_ATOMIC ? 1 : 0;
         rc = window_enqueue(c, alg, flags | SJB_F_C2, L, nL, R, nR, 1, out, out_cap, wrap,
-                            c->d_result, &logF, timing);
+                            c->d_result, &logF, timing ? c->ev : nullptr);
         if (rc) return rc;
         rc = fetch_result(c, &res);
         if (rc) return rc;
@@ -323,6 +324,9 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess)
         return fail(SJB_E_HIP);
+    c->ev[0] = c->ev0;
+    c->ev[1] = c->ev1;
+    c->ev[2] = c->ev2;
 
     const uint64_t cap = c->cap;
     if (c->opts.hbm_budget_bytes) {
@@ -368,6 +372,7 @@ int sjb_close(sjb_ctx_t *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    for (hipEvent_t e : c->tring) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SJB_OK;
@@ -491,8 +496,45 @@ int sjb_lazy_join_async(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_t
         !device_accessible(d_result))
         return SJB_E_INVALID;
     int logF;
-    return window_enqueue(c, algorithm, flags | c->opts.flags, d_left, n_left, d_right, n_right, 0,
-                          nullptr, 0, 0, d_result, &logF, false);
+    flags |= c->opts.flags;
+    const hipEvent_t *ev = nullptr;
+    if ((flags & SJB_F_TIMING) && !c->tring.empty()) {
+        ev = &c->tring[3 * (c->tring_next % (c->tring.size() / 3))];
+        c->tring_next++;
+    }
+    return window_enqueue(c, algorithm, flags, d_left, n_left, d_right, n_right, 0, nullptr, 0, 0,
+                          d_result, &logF, ev);
+}
+
+int sjb_timing_ring(sjb_ctx_t *c, uint32_t n) {
+    if (!c || n > (1u << 20)) return SJB_E_INVALID;
+    SJB_HIP_TRY(hipSetDevice(c->device));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (hipEvent_t e : c->tring) (void)hipEventDestroy(e);
+    c->tring.clear();
+    c->tring_next = 0;
+    for (uint32_t i = 0; i < 3 * n; i++) {
+        hipEvent_t e;
+        SJB_HIP_TRY(hipEventCreate(&e));
+        c->tring.push_back(e);
+    }
+    return SJB_OK;
+}
+
+int sjb_timing_read(sjb_ctx_t *c, float *part_ms, float *join_ms, uint32_t n, uint32_t *n_read) {
+    if (!c || !part_ms || !join_ms || !n_read) return SJB_E_INVALID;
+    SJB_HIP_TRY(hipSetDevice(c->device));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t slots = c->tring.size() / 3;
+    const uint64_t have = std::min<uint64_t>({uint64_t(n), slots, c->tring_next});
+    for (uint64_t i = 0; i < have; i++) {
+        const uint64_t w = c->tring_next - have + i;  // oldest first
+        const hipEvent_t *e = &c->tring[3 * (w % slots)];
+        SJB_HIP_TRY(hipEventElapsedTime(&part_ms[i], e[0], e[1]));
+        SJB_HIP_TRY(hipEventElapsedTime(&join_ms[i], e[1], e[2]));
+    }
+    *n_read = uint32_t(have);
+    return SJB_OK;
 }
 
 int sjb_lazy_join_batch(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_tuple_t *d_left,

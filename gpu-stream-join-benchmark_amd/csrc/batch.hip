@@ -343,7 +343,7 @@ int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const 
             int logF;
             rc = window_enqueue(c, alg, flags | c->opts.flags, L + loff[w], loff[w + 1] - loff[w],
                                 R + roff[w], roff[w + 1] - roff[w], 0, nullptr, 0, 0,
-                                c->d_batch_res + w, &logF, false);
+                                c->d_batch_res + w, &logF, nullptr);
             if (rc) return rc;
         }
     } else {

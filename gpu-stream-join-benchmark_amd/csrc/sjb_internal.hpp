@@ -182,6 +182,9 @@ struct sjb_ctx {
     uint64_t *d_cursor = nullptr;     // pair-output cursor
     uint64_t *d_heavy = nullptr;      // heavy-key aggregates
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev[3] = {};             // ev0, ev1, ev2 (per-phase timing of one call)
+    std::vector<hipEvent_t> tring;    // timing ring: 3 events per async window
+    uint64_t tring_next = 0;
 
     // device staging of pair output when the caller's buffer is pageable
     sjb_result_tuple_t *d_out_stage = nullptr;
@@ -263,7 +266,7 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                    const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
                    uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
-                   bool timing);
+                   const hipEvent_t *ev);  // 3 events (partition start / join start / end) or null
 
 // ---- batch.hip ----
 int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const uint64_t *loff,

@@ -121,6 +121,8 @@ SIGNATURES = {
                                 ctypes.POINTER(_u32)]),
     "sjb_batch_get_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
     "sjb_batch_set_tune": (_i, [_p, ctypes.POINTER(BatchTune)]),
+    "sjb_timing_ring": (_i, [_p, _u32]),
+    "sjb_timing_read": (_i, [_p, _p, _p, _u32, ctypes.POINTER(_u32)]),
     "sjb_rmi_fit": (_i, [_p, _p, _u64, ctypes.POINTER(RmiConfig), ctypes.POINTER(RmiReport)]),
     "sjb_rmi_clear": (_i, [_p]),
     "sjb_mlp_grad": (_i, [_p, _p, _p, _p, _u32, _p, ctypes.POINTER(_d)]),
@@ -292,6 +294,16 @@ class Context:
                 setattr(t, k, v)
             check(lib().sjb_batch_set_tune(self.h, ctypes.byref(t)), "sjb_batch_set_tune")
         return t
+
+    def timing_ring(self, n):
+        check(lib().sjb_timing_ring(self.h, n), "sjb_timing_ring")
+
+    def timing_read(self, n):
+        """(partition ms, join ms) arrays of the last n timed async windows."""
+        a, b = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+        got = ctypes.c_uint32()
+        check(lib().sjb_timing_read(self.h, ptr(a), ptr(b), n, ctypes.byref(got)), "sjb_timing_read")
+        return a[:got.value], b[:got.value]
 
     def rmi_fit(self, train, n_train, root=RMI_MLP, steps=400, lr=0.01, seed=3, heavy_min=3):
         cfg = RmiConfig(root, steps, lr, seed, heavy_min, 0)

@@ -178,6 +178,15 @@ int sjb_lazy_join_async(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
                         const sjb_tuple_t *d_right, uint64_t n_right,
                         sjb_window_result_t *d_result);
 
+/* Per-window device timing of sjb_lazy_join_async calls made with
+ * SJB_F_TIMING: sjb_timing_ring allocates a ring of n windows x 3 HIP events
+ * (partition start, join start, end; recorded on the ctx stream around the
+ * partition kernels and the join + reduce kernels), sjb_timing_read returns
+ * the last min(n, recorded) windows' partition and join milliseconds,
+ * oldest first (synchronises the ctx stream).                              */
+int sjb_timing_ring(sjb_ctx_t *ctx, uint32_t n);
+int sjb_timing_read(sjb_ctx_t *ctx, float *part_ms, float *join_ms, uint32_t n, uint32_t *n_read);
+
 /* Many uneven windows in one launch sequence (count/checksum only).  Window
  * w is d_left[left_off[w], left_off[w+1]) x d_right[right_off[w], right_off[w+1]);
  * offsets are host arrays of n_windows+1 entries.  Results (host array) are
