@@ -63,11 +63,12 @@ static int grow(sjb_batch_plan *p, T **ptr, size_t *cap, size_t need) {
     if (*ptr) {
         (void)hipFree(*ptr);
         p->dev_bytes -= *cap * sizeof(T);
+        p->c->ext_bytes -= *cap * sizeof(T);
         *ptr = nullptr;
         *cap = 0;
     }
     const uint64_t budget = p->c->opts.hbm_budget_bytes;
-    if (budget && p->c->bytes + p->dev_bytes + n * sizeof(T) > budget) return SJB_E_BUDGET;
+    if (budget && p->c->bytes + p->c->ext_bytes + n * sizeof(T) > budget) return SJB_E_BUDGET;
     if (hipMalloc(reinterpret_cast<void **>(ptr), n * sizeof(T)) != hipSuccess) {
         (void)hipGetLastError();
         *ptr = nullptr;
@@ -75,11 +76,13 @@ static int grow(sjb_batch_plan *p, T **ptr, size_t *cap, size_t need) {
     }
     *cap = n;
     p->dev_bytes += n * sizeof(T);
+    p->c->ext_bytes += n * sizeof(T);
     return SJB_OK;
 }
 
 void batch_plan_free(sjb_batch_plan *p) {
     if (!p) return;
+    if (p->c) p->c->ext_bytes -= p->dev_bytes;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     void *dev[] = {p->d_win, p->d_chunk_win, p->d_item_win, p->d_part, p->d_offs, p->d_qhead, p->d_heavy};

@@ -155,6 +155,7 @@ struct sjb_ctx {
     sjb_options_t opts{};
     int n_cu = 0;
     uint64_t bytes = 0;
+    uint64_t ext_bytes = 0;           // device bytes of live batch plans and sliding windows
     std::vector<void *> dev_allocs;
     std::vector<void *> host_allocs;
 

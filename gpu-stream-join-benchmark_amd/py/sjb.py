@@ -95,6 +95,14 @@ class PartitionReport(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class SlideInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("window", "max_batch", "table_slots", "device_bytes",
+                                               "events", "rebuilds", "tuples_left", "tuples_right")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 RMI_LINEAR, RMI_MLP = 0, 1
 MLP_PARAMS = 1153
 
@@ -128,6 +136,12 @@ SIGNATURES = {
     "sjb_mlp_grad": (_i, [_p, _p, _p, _p, _u32, _p, ctypes.POINTER(_d)]),
     "sjb_partition_report": (_i, [_p, _i, _u32, _p, _u64, _p, _u64, ctypes.POINTER(PartitionReport)]),
     "sjb_generate_mix": (_i, [_p, _u64, _u64, _d, _d, _u64, _u64]),
+    "sjb_slide_create": (_i, [_p, _u64, _u64, ctypes.POINTER(_p)]),
+    "sjb_slide_reset": (_i, [_p]),
+    "sjb_slide_push": (_i, [_p, _p, _u64, _p, _u64, _p]),
+    "sjb_slide_serve": (_i, [_p, _p, _p, _p, _p, _u32, _p, _p, _p, ctypes.POINTER(_d)]),
+    "sjb_slide_info": (_i, [_p, ctypes.POINTER(SlideInfo)]),
+    "sjb_slide_destroy": (_i, [_p]),
     "sjb_eager_execute": (_i, [_p, _i, _i, _u32, _p, _u64, _i, _p, _u64, ctypes.POINTER(Stats)]),
     "sjb_eager_clear": (_i, [_p]),
     "sjb_eager_end_of_stream": (_i, [_p]),
@@ -276,6 +290,15 @@ class Context:
         self._plans.append(plan)
         return plan
 
+    def slide(self, window, max_batch):
+        """Creates a sliding-window join state (raises SjbError SJB_E_BUDGET when it
+        would not fit the ctx's HBM budget)."""
+        h = ctypes.c_void_p()
+        check(lib().sjb_slide_create(self.h, window, max_batch, ctypes.byref(h)), "sjb_slide_create")
+        sl = Slide(h, self)
+        self._plans.append(sl)
+        return sl
+
     def batch_autotune(self, alg, flags, left, left_off, right, right_off, reps=3):
         lo = np.ascontiguousarray(left_off, dtype=np.uint64)
         ro = np.ascontiguousarray(right_off, dtype=np.uint64)
@@ -375,6 +398,53 @@ class BatchPlan:
     def destroy(self):
         if self.h:
             lib().sjb_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Slide:
+    """Sliding-window join state (sjb_slide_*)."""
+
+    def __init__(self, h, ctx):
+        self.h = h
+        self.ctx = ctx
+
+    def push(self, left, right):
+        r = np.zeros(1, dtype=WRES_DTYPE)
+        check(lib().sjb_slide_push(self.h, ptr(left) if len(left) else None, len(left),
+                                   ptr(right) if len(right) else None, len(right), ptr(r)),
+              "sjb_slide_push")
+        return int(r[0]["count"]), int(r[0]["checksum1"])
+
+    def serve(self, left, left_off, right, right_off, arrival_s):
+        """Runs the serving loop; returns (results, latency_s, elapsed_s)."""
+        lo = np.ascontiguousarray(left_off, dtype=np.uint64)
+        ro = np.ascontiguousarray(right_off, dtype=np.uint64)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        n = len(lo) - 1
+        lat = np.zeros(n, dtype=np.float64)
+        res = np.zeros(n, dtype=WRES_DTYPE)
+        el = ctypes.c_double()
+        check(lib().sjb_slide_serve(self.h, ptr(left), ptr(lo), ptr(right), ptr(ro), n, ptr(arr), ptr(lat),
+                                    ptr(res), ctypes.byref(el)), "sjb_slide_serve")
+        return res, lat, el.value
+
+    def reset(self):
+        check(lib().sjb_slide_reset(self.h), "sjb_slide_reset")
+
+    def info(self):
+        i = SlideInfo()
+        check(lib().sjb_slide_info(self.h, ctypes.byref(i)), "sjb_slide_info")
+        return i
+
+    def destroy(self):
+        if self.h:
+            lib().sjb_slide_destroy(self.h)
             self.h = None
 
     def __del__(self):

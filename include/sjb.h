@@ -308,6 +308,43 @@ int sjb_partition_report(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
                          const sjb_tuple_t *d_right, uint64_t n_right,
                          sjb_partition_report_t *report);
 
+/* ---- eager sliding-window join (device-resident state, serving loop) --- */
+/* A sliding count window over two streams: after every event (one left and
+ * one right batch, either may be empty) the window holds the last `window`
+ * tuples of each side, and the event's result is count and C1 of that
+ * window's join.  State stays on the device (per side: a ring of `window`
+ * tuples and a group-by-key table of count and sum of A / B); an event costs
+ * O(batch) lookups whatever the window size (expired and new tuples are
+ * probed against the other side, then applied to their own).  Generalises
+ * the reference's eager executors (BaseEagerExecutor::execute per batch,
+ * include/sjb/windowing/eager/executor/BaseEagerExecutor.hpp:16-20; state
+ * cleared per tumbling window by include/sjb/windowing/eager/CountBased.cuh:36-72)
+ * to a sliding window.  Admission: sjb_slide_create returns SJB_E_BUDGET
+ * when the state would exceed the ctx's hbm_budget_bytes together with the
+ * ctx, its batch plans and its other sliding windows.                      */
+typedef struct sjb_slide sjb_slide_t;
+typedef struct sjb_slide_info {
+    uint64_t window, max_batch, table_slots, device_bytes;
+    uint64_t events, rebuilds, tuples_left, tuples_right;
+} sjb_slide_info_t;
+int sjb_slide_create(sjb_ctx_t *ctx, uint64_t window, uint64_t max_batch, sjb_slide_t **slide);
+int sjb_slide_reset(sjb_slide_t *slide);
+/* One event from host batches (at most max_batch tuples per side); returns
+ * when the window's result is host-visible.                                 */
+int sjb_slide_push(sjb_slide_t *slide, const sjb_tuple_t *left, uint64_t n_left,
+                   const sjb_tuple_t *right, uint64_t n_right, sjb_window_result_t *result);
+/* Serving loop: event i = h_left[left_off[i], left_off[i+1]) and
+ * h_right[right_off[i], right_off[i+1]) arrives arrival_s[i] seconds after
+ * the call starts and is enqueued once it has arrived; latency_s[i] = time
+ * its result became host-visible minus its arrival.  Host batches should be
+ * pinned (hipHostMalloc) for asynchronous copies.                           */
+int sjb_slide_serve(sjb_slide_t *slide, const sjb_tuple_t *h_left, const uint64_t *left_off,
+                    const sjb_tuple_t *h_right, const uint64_t *right_off, uint32_t n_events,
+                    const double *arrival_s, double *latency_s, sjb_window_result_t *results,
+                    double *elapsed_s);
+int sjb_slide_info(sjb_slide_t *slide, sjb_slide_info_t *info);
+int sjb_slide_destroy(sjb_slide_t *slide);
+
 /* ---- eager (per-batch) join -------------------------------------------- */
 /* Joins one incoming batch against the opposite side's resident window state
  * and then appends it to its own side's state.  Replaces

@@ -57,6 +57,7 @@ def lib():
             "or_sliding": (_i, [_p, _p, _u64, _u64, _u64, _p]),
             "or_bench_lazy_hj": (ctypes.c_double, [_p, _p, _u64, _u64, _u64, _i, _R]),
             "or_mlp_grad": (ctypes.c_double, [_p, _p, _p, _u32, _p]),
+            "or_sliding_var": (_i, [_p, _p, _p, _p, _u64, _u64, _p]),
             "or_mlp_loss": (ctypes.c_double, [_p, _p, _p, _u32]),
         }
         for name, (res, args) in sig.items():
@@ -132,6 +133,18 @@ def sliding(R, S, W, slide):
     rc = lib().or_sliding(_ptr(R), _ptr(S), n, W, slide, ctypes.cast(res, ctypes.c_void_p))
     assert rc == 0
     return [res[i].triple() for i in range(nw)]
+
+
+def sliding_var(R, endR, S, endS, W):
+    """(count, C1) of the last-W-tuples window of each side after every event."""
+    endR = np.ascontiguousarray(endR, dtype=np.uint64)
+    endS = np.ascontiguousarray(endS, dtype=np.uint64)
+    n = len(endR)
+    res = (Result * max(n, 1))()
+    rc = lib().or_sliding_var(_ptr(R), endR.ctypes.data, _ptr(S), endS.ctypes.data, n, W,
+                              ctypes.cast(res, ctypes.c_void_p))
+    assert rc == 0
+    return [(res[i].count, res[i].c1) for i in range(n)]
 
 
 def bench_lazy_hj(R, S, W, D, threads):

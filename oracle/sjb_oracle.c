@@ -283,6 +283,16 @@ int or_sliding(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uin
     return 0;
 }
 
+int or_sliding_var(const or_tuple *R, const uint64_t *endR, const or_tuple *S, const uint64_t *endS,
+                   uint64_t n_events, uint64_t W, or_result *res) {
+    for (uint64_t i = 0; i < n_events; i++) {
+        const uint64_t bR = endR[i] > W ? endR[i] - W : 0, bS = endS[i] > W ? endS[i] - W : 0;
+        int rc = or_lazy_smj(R + bR, endR[i] - bR, S + bS, endS[i] - bS, 0, &res[i], NULL, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Timed CPU baseline: the lazy CPU_HJ over tumbling windows of W tuples   */
 /* per side with the INTRA decomposition (1 joiner, `threads` probe        */

@@ -76,6 +76,12 @@ int or_eager_execute(or_eager *e, const or_tuple *batch, uint64_t n, int is_left
 int or_sliding(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uint64_t slide,
                or_result *res);
 
+/* Sliding count window with batches of varying size: after event i the
+ * window of each side is its last W tuples, R[max(0, endR[i]-W), endR[i])
+ * and likewise for S.  Count and C1 per event (closed form per key).      */
+int or_sliding_var(const or_tuple *R, const uint64_t *endR, const or_tuple *S, const uint64_t *endS,
+                   uint64_t n_events, uint64_t W, or_result *res);
+
 /* Timed CPU baseline: lazy hash join over consecutive tumbling windows of W
  * tuples with `threads` probe threads (INTRA decomposition of
  * executable/runExperiment.cpp:163-166).  Returns elapsed seconds.        */
