@@ -95,8 +95,12 @@ static bool valid_method(int m) { return m >= SJB_OUT_NOOUTPUT && m <= SJB_OUT_A
 // partition pass (plus the range parameters or the heavy-key reset it
 // needs).  Both sides non-empty.
 int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
-                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out) {
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out, int buf) {
     WindowPlan &p = *out;
+    p.ps = buf < 0 ? c->stream : c->ov_stream;
+    p.part = buf < 0 ? c->d_part : c->ov_part[buf];
+    p.offs = buf < 0 ? c->d_offs : c->ov_offs[buf];
+    p.heavy = buf < 0 ? c->d_heavy : c->ov_heavy[buf];
     p.alg = alg;
     p.flags = flags;
     p.nL = nL;
@@ -111,14 +115,35 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
     int rc;
     if (p.kind == PART_RANGE) {
-        rc = launch_minmax(c, L, nL, R, nR, p.logF);
+        rc = launch_minmax(c, p.ps, L, nL, R, nR, p.logF);
         if (rc) return rc;
     }
     if (p.peel) {
-        rc = launch_fill_u64(c->stream, c->d_heavy, 0, 4 * kMaxHeavy);
+        rc = launch_fill_u64(p.ps, p.heavy, 0, 4 * kMaxHeavy);
         if (rc) return rc;
     }
     return launch_partition(c, p, L, R);
+}
+
+// Second buffer set, stream and events of overlapped async windows
+// (allocated on first use).
+static int overlap_setup(sjb_ctx *c) {
+    if (c->ov_stream) return SJB_OK;
+    int rc;
+    if ((rc = dev_alloc(c, (void **)&c->ov_part[1], 2 * c->cap * sizeof(Rec))) ||
+        (rc = dev_alloc(c, (void **)&c->ov_offs[1], c->offs_cap * sizeof(uint32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->ov_heavy[1], 4 * kMaxHeavy * sizeof(uint64_t))))
+        return rc;
+    c->ov_part[0] = c->d_part;
+    c->ov_offs[0] = c->d_offs;
+    c->ov_heavy[0] = c->d_heavy;
+    for (int b = 0; b < 2; b++) {
+        SJB_HIP_TRY(hipEventCreateWithFlags(&c->ov_part_done[b], hipEventDisableTiming));
+        SJB_HIP_TRY(hipEventCreateWithFlags(&c->ov_join_done[b], hipEventDisableTiming));
+        SJB_HIP_TRY(hipEventRecord(c->ov_join_done[b], c->stream));
+    }
+    SJB_HIP_TRY(hipStreamCreateWithFlags(&c->ov_stream, hipStreamNonBlocking));
+    return SJB_OK;
 }
 
 // Enqueues one window (count/checksums or pair output) on the ctx stream;
@@ -126,7 +151,7 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                    const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
                    uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
-                   const hipEvent_t *ev) {
+                   const hipEvent_t *ev, int buf) {
     int rc;
     if (write) {
         rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
@@ -137,26 +162,41 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
         return launch_fill_u64(c->stream, reinterpret_cast<uint64_t *>(d_res), 0,
                                sizeof(sjb_window_result_t) / 8);
     }
-    if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], c->stream));
     if (alg == SJB_ALG_NLJ) {
         *logF_used = 0;
+        if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], c->stream));
         if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], c->stream));
+        if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
         rc = launch_nlj(c, L, nL, R, nR, flags, write, out, out_cap, wrap, d_res);
         if (rc) return rc;
-        if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
+        if (ev) SJB_HIP_TRY(hipEventRecord(ev[3], c->stream));
         return SJB_OK;
     }
+    // overlapped: the partition pass waits for the last join that read this
+    // buffer set, and the join waits for the partition pass
+    hipStream_t ps = buf < 0 ? c->stream : c->ov_stream;
+    if (buf >= 0) SJB_HIP_TRY(hipStreamWaitEvent(ps, c->ov_join_done[buf], 0));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], ps));
     WindowPlan p;
-    rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p);
+    rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p, buf);
     if (rc) return rc;
     *logF_used = p.logF;
-    if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], c->stream));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], ps));
+    if (buf >= 0) {
+        SJB_HIP_TRY(hipEventRecord(c->ov_part_done[buf], ps));
+        SJB_HIP_TRY(hipStreamWaitEvent(c->stream, c->ov_part_done[buf], 0));
+    }
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res, p.peel ? c->rmi_nheavy : 0);
+    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res, p.peel ? c->rmi_nheavy : 0,
+                       p.heavy);
     if (rc) return rc;
-    if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
+    if (ev) SJB_HIP_TRY(hipEventRecord(ev[3], c->stream));
+    // buffer set 0 is also the ctx's own: its next overlapped partition pass
+    // waits for this join
+    if (c->ov_stream) SJB_HIP_TRY(hipEventRecord(c->ov_join_done[buf < 0 ? 0 : buf], c->stream));
     return SJB_OK;
 }
 
@@ -171,8 +211,8 @@ static int fetch_result(sjb_ctx *c, sjb_window_result_t *host) {
 static void fill_times(sjb_ctx *c, int alg, bool timing, sjb_stats_t *st) {
     if (!timing) return;
     float a = 0.f, b = 0.f;
-    if (hipEventElapsedTime(&a, c->ev0, c->ev1) != hipSuccess) a = 0.f;
-    if (hipEventElapsedTime(&b, c->ev1, c->ev2) != hipSuccess) b = 0.f;
+    if (hipEventElapsedTime(&a, c->ev[0], c->ev[1]) != hipSuccess) a = 0.f;
+    if (hipEventElapsedTime(&b, c->ev[2], c->ev[3]) != hipSuccess) b = 0.f;
     const uint64_t ua = uint64_t(a * 1000.f), ub = uint64_t(b * 1000.f);
     st->device_ns += uint64_t((a + b) * 1.0e6f);
     if (alg == SJB_ALG_SMJ) {
@@ -322,11 +362,12 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SJB_E_HIP);
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess)
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
         return fail(SJB_E_HIP);
     c->ev[0] = c->ev0;
     c->ev[1] = c->ev1;
-    c->ev[2] = c->ev2;
+    c->ev[2] = c->ev1;  // one stream: partition end = join start
+    c->ev[3] = c->ev2;
 
     const uint64_t cap = c->cap;
     if (c->opts.hbm_budget_bytes) {
@@ -372,7 +413,16 @@ int sjb_close(sjb_ctx_t *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     for (hipEvent_t e : c->tring) (void)hipEventDestroy(e);
+    if (c->ov_stream) {
+        (void)hipStreamSynchronize(c->ov_stream);
+        (void)hipStreamDestroy(c->ov_stream);
+    }
+    for (int b = 0; b < 2; b++) {
+        if (c->ov_part_done[b]) (void)hipEventDestroy(c->ov_part_done[b]);
+        if (c->ov_join_done[b]) (void)hipEventDestroy(c->ov_join_done[b]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SJB_OK;
@@ -382,6 +432,7 @@ void *sjb_stream(sjb_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
 
 int sjb_sync(sjb_ctx_t *c) {
     if (!c) return SJB_E_INVALID;
+    if (c->ov_stream) SJB_HIP_TRY(hipStreamSynchronize(c->ov_stream));
     SJB_HIP_TRY(hipStreamSynchronize(c->stream));
     return SJB_OK;
 }
@@ -499,11 +550,18 @@ int sjb_lazy_join_async(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_t
     flags |= c->opts.flags;
     const hipEvent_t *ev = nullptr;
     if ((flags & SJB_F_TIMING) && !c->tring.empty()) {
-        ev = &c->tring[3 * (c->tring_next % (c->tring.size() / 3))];
+        ev = &c->tring[4 * (c->tring_next % (c->tring.size() / 4))];
         c->tring_next++;
     }
+    int buf = -1;
+    if ((flags & SJB_F_OVERLAP) && algorithm != SJB_ALG_NLJ) {
+        SJB_HIP_TRY(hipSetDevice(c->device));
+        int rc = overlap_setup(c);
+        if (rc) return rc;
+        buf = int(c->ov_next++ & 1);
+    }
     return window_enqueue(c, algorithm, flags, d_left, n_left, d_right, n_right, 0, nullptr, 0, 0,
-                          d_result, &logF, ev);
+                          d_result, &logF, ev, buf);
 }
 
 int sjb_timing_ring(sjb_ctx_t *c, uint32_t n) {
@@ -513,7 +571,7 @@ int sjb_timing_ring(sjb_ctx_t *c, uint32_t n) {
     for (hipEvent_t e : c->tring) (void)hipEventDestroy(e);
     c->tring.clear();
     c->tring_next = 0;
-    for (uint32_t i = 0; i < 3 * n; i++) {
+    for (uint32_t i = 0; i < 4 * n; i++) {
         hipEvent_t e;
         SJB_HIP_TRY(hipEventCreate(&e));
         c->tring.push_back(e);
@@ -525,13 +583,13 @@ int sjb_timing_read(sjb_ctx_t *c, float *part_ms, float *join_ms, uint32_t n, ui
     if (!c || !part_ms || !join_ms || !n_read) return SJB_E_INVALID;
     SJB_HIP_TRY(hipSetDevice(c->device));
     SJB_HIP_TRY(hipStreamSynchronize(c->stream));
-    const uint64_t slots = c->tring.size() / 3;
+    const uint64_t slots = c->tring.size() / 4;
     const uint64_t have = std::min<uint64_t>({uint64_t(n), slots, c->tring_next});
     for (uint64_t i = 0; i < have; i++) {
         const uint64_t w = c->tring_next - have + i;  // oldest first
-        const hipEvent_t *e = &c->tring[3 * (w % slots)];
+        const hipEvent_t *e = &c->tring[4 * (w % slots)];
         SJB_HIP_TRY(hipEventElapsedTime(&part_ms[i], e[0], e[1]));
-        SJB_HIP_TRY(hipEventElapsedTime(&join_ms[i], e[1], e[2]));
+        SJB_HIP_TRY(hipEventElapsedTime(&join_ms[i], e[2], e[3]));
     }
     *n_read = uint32_t(have);
     return SJB_OK;

@@ -42,6 +42,8 @@ constexpr int kHjNB = 1 << kHjLogNB;
 constexpr int kBatch = 8;
 constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
 constexpr int kPartialsPerPart = kJoinThreads / kWave;  // single-window partials per partition
+constexpr int kHjProbe = 2048;                          // probe records staged in LDS per pass
+constexpr int kHjPerP = kHjProbe / kJoinThreads;        // staged probe records per thread
 
 // Record index of element j (< pref[nc]) by binary search over the run
 // prefix (used only past the run-id table).
@@ -271,31 +273,87 @@ __device__ __forceinline__ void pair_acc(uint32_t k, uint32_t rv, uint32_t sv, A
     if (C2) acc.c2 += pair_hash_from_a(a, sv);
 }
 
-// count / C1 / C2 of one probe record (payload = val).  The bucket scan
-// only compares keys and keeps the first two matching build payloads in
-// registers; the 64-bit checksum math runs once per probe after the scan
-// (a wave executes it once for all its matching lanes instead of once per
-// step).  Later matches in one bucket are handled inline.
+// Matches of one wave waiting for their checksum math: the index of the
+// probe record in the staging buffer and the build payload.  Queuing them
+// and running the 64-bit math 64 matches at a time keeps every lane busy,
+// where running it in place costs a full wave pass for each probe round in
+// which any lane matched.
+constexpr int kPairQ = 128;
+struct PairQ {
+    uint32_t j[kPairQ];
+    uint32_t rv[kPairQ];
+};
+
+// Lanes [0, n) (n <= 64) each take one queued match.
 template <bool C2>
-__device__ __forceinline__ void hj_probe_val(const HjTable &t, Rec s, int bshift, Acc &acc) {
-    const uint32_t b = hj_bucket(s.x, bshift);
-    const uint32_t lo = t.off[b], hi = t.off[b + 1];
-    uint32_t m = 0, rv0 = 0, rv1 = 0;
-    for (uint32_t e = lo; e < hi; e++) {
-        if (t.key[e] == s.x) {
-            const uint32_t rv = t.pay[e];
-            if (m == 0)
-                rv0 = rv;
-            else if (m == 1)
-                rv1 = rv;
-            else
-                pair_acc<C2>(s.x, rv, s.y, acc);
-            m++;
+__device__ __forceinline__ void pairq_drain(const PairQ &q, const Rec *sp, uint32_t n, Acc &acc) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane < n) {
+        const Rec s = sp[q.j[lane]];
+        pair_acc<C2>(s.x, q.rv[lane], s.y, acc);
+    }
+}
+
+// Count / C1 / C2 of the un probe records staged in LDS against the
+// current tile (payload = val).  The bucket scan compares keys only; the
+// first two matches of every probe go to the wave's queue (later matches,
+// rare, are summed in place).
+template <bool C2>
+__device__ __forceinline__ void hj_probe_compact(const HjTable &t, const Rec *sp, uint32_t un, int bshift,
+                                                 Acc &acc, PairQ &q) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t qn = 0;  // queued matches (wave-uniform)
+    for (uint32_t j0 = 0; j0 < un; j0 += kJoinThreads) {
+        const uint32_t j = j0 + threadIdx.x;
+        uint32_t m = 0, rv0 = 0, rv1 = 0;
+        if (j < un) {
+            const Rec s = sp[j];
+            const uint32_t b = hj_bucket(s.x, bshift);
+            const uint32_t lo = t.off[b], hi = t.off[b + 1];
+            for (uint32_t e = lo; e < hi; e++) {
+                if (t.key[e] == s.x) {
+                    const uint32_t rv = t.pay[e];
+                    if (m == 0)
+                        rv0 = rv;
+                    else if (m == 1)
+                        rv1 = rv;
+                    else
+                        pair_acc<C2>(s.x, rv, s.y, acc);
+                    m++;
+                }
+            }
+        }
+        acc.count += m;
+        for (uint32_t r = 0; r < 2; r++) {
+            const bool has = m > r;
+            const uint64_t mask = __ballot(has);
+            if (mask == 0) break;  // wave-uniform
+            if (has) {
+                const uint32_t pos = qn + lanes_below(mask);
+                q.j[pos] = j;
+                q.rv[pos] = r == 0 ? rv0 : rv1;
+            }
+            qn += uint32_t(__popcll(mask));
+            if (qn >= uint32_t(kWave)) {
+                pairq_drain<C2>(q, sp, kWave, acc);
+                // entries [64, qn) move down to [0, qn - 64)
+                uint32_t mj = 0, mr = 0;
+                const bool mv = lane < qn - kWave;
+                if (mv) {
+                    mj = q.j[kWave + lane];
+                    mr = q.rv[kWave + lane];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                if (mv) {
+                    q.j[lane] = mj;
+                    q.rv[lane] = mr;
+                }
+                qn -= kWave;
+            }
         }
     }
-    acc.count += m;
-    if (m >= 1) pair_acc<C2>(s.x, rv0, s.y, acc);
-    if (m >= 2) pair_acc<C2>(s.x, rv1, s.y, acc);
+    pairq_drain<C2>(q, sp, qn, acc);
 }
 
 // Probes the un records staged in LDS against the current tile.  WRITE
@@ -307,10 +365,9 @@ __device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, ui
                                              const sjb_tuple_t *__restrict__ R,
                                              sjb_result_tuple_t *__restrict__ out, uint64_t cap, int wrap,
                                              unsigned long long *__restrict__ cursor, uint32_t *ws,
-                                             unsigned long long *s_base) {
+                                             unsigned long long *s_base, PairQ &q) {
     if (!WRITE) {
-#pragma unroll 2
-        for (uint32_t j = threadIdx.x; j < un; j += kJoinThreads) hj_probe_val<C2>(t, sp[j], bshift, acc);
+        hj_probe_compact<C2>(t, sp, un, bshift, acc, q);
         return;
     }
     for (uint32_t j0 = 0; j0 < un; j0 += kJoinThreads) {
@@ -357,10 +414,11 @@ struct JoinIO {
 // LDS of the HJ body: the CSR build table and the probe staging buffer.
 struct HjLds {
     HjTable t;
-    Rec probe[kHjCap];
+    Rec probe[kHjProbe];
+    PairQ q[kJoinThreads / kWave];
 };
 
-// Joins one partition (HJ).  The first build tile and the first kHjCap
+// Joins one partition (HJ).  The first build tile and the first kHjProbe
 // probe records are gathered with all loads in flight at once (build
 // records go straight into the LDS table, probe records into an LDS
 // staging buffer); larger partitions continue tile by tile.  The caller
@@ -375,22 +433,23 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
     Rec *s_probe = m.probe;
     const Rec *__restrict__ part = io.part;
     for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
-    const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjCap);
+    PairQ &q = m.q[threadIdx.x / kWave];
+    const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjProbe);
     Rec rb[kHjPerT];
     {
-        Rec rp[kHjPerT];
+        Rec rp[kHjPerP];
 #pragma unroll
         for (int k = 0; k < kHjPerT; k++) {
             const uint32_t i = k * kJoinThreads + threadIdx.x;
             if (i < tn0) rb[k] = part[v.index(0, i)];
         }
 #pragma unroll
-        for (int k = 0; k < kHjPerT; k++) {
+        for (int k = 0; k < kHjPerP; k++) {
             const uint32_t j = k * kJoinThreads + threadIdx.x;
             if (j < pn0) rp[k] = part[v.index(1, j)];
         }
 #pragma unroll
-        for (int k = 0; k < kHjPerT; k++) {
+        for (int k = 0; k < kHjPerP; k++) {
             const uint32_t j = k * kJoinThreads + threadIdx.x;
             if (j < pn0) s_probe[j] = rp[k];
         }
@@ -408,8 +467,8 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
             }
         }
         hj_build(t, rb, tn, bshift, s_ws);
-        for (uint32_t u0 = 0; u0 < np; u0 += kHjCap) {
-            const uint32_t un = min<uint32_t>(kHjCap, np - u0);
+        for (uint32_t u0 = 0; u0 < np; u0 += kHjProbe) {
+            const uint32_t un = min<uint32_t>(kHjProbe, np - u0);
             if (u0 != resident) {
                 __syncthreads();
                 gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { s_probe[i] = r; });
@@ -417,7 +476,7 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
                 __syncthreads();
             }
             hj_probe_lds<C2, WRITE>(t, s_probe, un, bshift, acc, io.L, io.R, io.out, io.out_cap,
-                                    io.wrap, io.cursor, s_ws, s_base);
+                                    io.wrap, io.cursor, s_ws, s_base, q);
         }
         __syncthreads();
     }
@@ -964,8 +1023,9 @@ __global__ __launch_bounds__(1024) void k_reduce(const uint64_t *__restrict__ pa
     }
 }
 
-int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy) {
-    k_reduce<<<1, 1024, 0, c->stream>>>(c->d_partials, n_partials, c->d_heavy, n_heavy, d_out);
+int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy,
+                  const uint64_t *heavy) {
+    k_reduce<<<1, 1024, 0, c->stream>>>(c->d_partials, n_partials, heavy ? heavy : c->d_heavy, n_heavy, d_out);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -983,7 +1043,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     const int bshift = p.kind == PART_HASH ? p.logF : 0;
     const size_t dynb = part_view_bytes(p.ncL, p.ncR, idcap_of<ALG>());
     JoinIO io;
-    io.part = c->d_part;
+    io.part = p.part;
     io.L = L;
     io.R = R;
     io.out = out;
@@ -991,7 +1051,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     io.wrap = wrap;
     io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
 #define SJB_J(C2_, W_, G_)                                                                          \
-    k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, c->d_offs, p.logF, p.ncL,   \
+    k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL,      \
                                                                  p.ncR, uint32_t(p.nL), bshift,    \
                                                                  c->d_partials)
     if (p.payload == PAY_IDX) {

@@ -164,15 +164,15 @@ __global__ void k_minmax_final(const uint32_t *__restrict__ partial, int nblocks
     pp->shift = bits > logF ? uint32_t(bits - logF) : 0u;
 }
 
-int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
+int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
                   uint64_t nR, int logF) {
     uint64_t n = nL + nR;
     int blocks = int(std::min<uint64_t>((n + kChunk - 1) / kChunk, 1024));
     if (blocks < 1) blocks = 1;
     // The scan-block buffer doubles as scratch for the partial min/max.
-    k_minmax_partial<<<blocks, kPartThreads, 0, c->stream>>>(L, nL, R, nR, c->d_scratch);
+    k_minmax_partial<<<blocks, kPartThreads, 0, s>>>(L, nL, R, nR, c->d_scratch);
     SJB_HIP_TRY(hipGetLastError());
-    k_minmax_final<<<1, 64, 0, c->stream>>>(c->d_scratch, blocks, logF, c->d_pparams);
+    k_minmax_final<<<1, 64, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -447,11 +447,11 @@ static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
     const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;
     if (p.payload == PAY_VAL)
-        k_partition<KIND, PAY_VAL, PEEL><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part, c->d_heavy);
+        k_partition<KIND, PAY_VAL, PEEL><<<nblk, kPartThreads, lds, p.ps>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy);
     else
-        k_partition<KIND, PAY_IDX, false><<<nblk, kPartThreads, lds, c->stream>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, c->d_offs, c->d_part, c->d_heavy);
+        k_partition<KIND, PAY_IDX, false><<<nblk, kPartThreads, lds, p.ps>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }

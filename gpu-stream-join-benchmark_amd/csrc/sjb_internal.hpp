@@ -145,6 +145,12 @@ struct WindowPlan {
     uint64_t nL, nR;
     int ncL, ncR;  // chunks per side
     bool peel;     // learned + grouped: heavy keys aggregated apart
+    // where the partition pass runs and writes (the ctx's own buffers and
+    // stream, or the second set of an overlapped window)
+    hipStream_t ps;
+    Rec *part;
+    uint32_t *offs;
+    uint64_t *heavy;
 };
 
 }  // namespace sjb
@@ -182,10 +188,20 @@ struct sjb_ctx {
     uint32_t loss_hist_cap = 0;
     uint64_t *d_cursor = nullptr;     // pair-output cursor
     uint64_t *d_heavy = nullptr;      // heavy-key aggregates
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    hipEvent_t ev[3] = {};             // ev0, ev1, ev2 (per-phase timing of one call)
-    std::vector<hipEvent_t> tring;    // timing ring: 3 events per async window
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    hipEvent_t ev[4] = {};             // per-phase timing of one call (partition start / end, join start / end)
+    std::vector<hipEvent_t> tring;    // timing ring: 4 events per async window
     uint64_t tring_next = 0;
+    // overlapped async windows (SJB_F_OVERLAP): the partition pass of a
+    // window runs on ov_stream into buffer set (window index & 1) while the
+    // previous window's join runs on the ctx stream; set 0 is the ctx's own
+    // d_part / d_offs / d_heavy
+    hipStream_t ov_stream = nullptr;
+    sjb::Rec *ov_part[2] = {};
+    uint32_t *ov_offs[2] = {};
+    uint64_t *ov_heavy[2] = {};
+    hipEvent_t ov_part_done[2] = {}, ov_join_done[2] = {};
+    uint64_t ov_next = 0;
 
     // device staging of pair output when the caller's buffer is pageable
     sjb_result_tuple_t *d_out_stage = nullptr;
@@ -237,8 +253,8 @@ int host_alloc(sjb_ctx *c, void **p, size_t bytes);
 
 // ---- launchers (partition.hip) ----
 int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested);
-int launch_minmax(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
-                  int logF);
+int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
+                  uint64_t nR, int logF);
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
 int partition_setup(sjb_ctx *c);
 int launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, uint64_t n);
@@ -249,7 +265,8 @@ int launch_hj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const 
                    sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
 int launch_smj_join(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
                     sjb_result_tuple_t *out, uint64_t out_cap, int wrap);
-int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy);
+int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32_t n_heavy,
+                  const uint64_t *heavy = nullptr);
 
 // ---- launchers (nlj.hip) ----
 int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
@@ -263,11 +280,12 @@ int launch_part_sizes(hipStream_t s, const uint32_t *offs, int logF, int ncL, in
 
 // ---- api.hip ----
 int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
-                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *p);
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *p, int buf = -1);
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                    const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
                    uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
-                   const hipEvent_t *ev);  // 3 events (partition start / join start / end) or null
+                   const hipEvent_t *ev, int buf = -1);  // ev: 4 events (partition start / end, join
+                                                         // start / end) or null; buf: overlapped buffer set
 
 // ---- batch.hip ----
 int batch_join(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, const uint64_t *loff,

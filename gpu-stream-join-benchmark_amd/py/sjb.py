@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("SJB_LIB") or os.path.join(PKG, "lib", "libsjb_hip.so"
 # enums (include/sjb/windowing/BaseWindowing.cuh:9-40 values)
 ALG = {"EMPTY": 0, "NLJ": 1, "HJ": 2, "SMJ": 3}
 OUT = {"NoOutput": 0, "CountKernel": 1, "EstimatedSelectivity": 2, "Bitmap": 3, "Atomic": 4}
-F_C2, F_GROUPED, F_LEARNED, F_TIMING = 0x1, 0x2, 0x4, 0x8
+F_C2, F_GROUPED, F_LEARNED, F_TIMING, F_OVERLAP = 0x1, 0x2, 0x4, 0x8, 0x10
 DIST_UNIFORM, DIST_ZIPF = 0, 1
 
 E = {

@@ -70,6 +70,10 @@ extern "C" {
                                /* O(n) at any skew, C2 not produced            */
 #define SJB_F_LEARNED 0x4u     /* partition with the ctx's learned CDF model   */
 #define SJB_F_TIMING 0x8u      /* record per-phase device time (hipEvents)     */
+#define SJB_F_OVERLAP 0x10u    /* sjb_lazy_join_async: the window's partition  */
+                               /* pass may run on a second stream, overlapping */
+                               /* the previous window's join; the inputs must  */
+                               /* be ready when the call is made               */
 
 /* ---- data layout ------------------------------------------------------- */
 /* Identical to Tuple (include/sjb/utils/Tuple.hpp:6-11): 16 B, AoS.        */
@@ -179,11 +183,12 @@ int sjb_lazy_join_async(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
                         sjb_window_result_t *d_result);
 
 /* Per-window device timing of sjb_lazy_join_async calls made with
- * SJB_F_TIMING: sjb_timing_ring allocates a ring of n windows x 3 HIP events
- * (partition start, join start, end; recorded on the ctx stream around the
- * partition kernels and the join + reduce kernels), sjb_timing_read returns
- * the last min(n, recorded) windows' partition and join milliseconds,
- * oldest first (synchronises the ctx stream).                              */
+ * SJB_F_TIMING: sjb_timing_ring allocates a ring of n windows x 4 HIP events
+ * (partition start / end, join start / end, recorded on the stream of each
+ * phase around the partition kernels and the join + reduce kernels);
+ * sjb_timing_read returns the last min(n, recorded) windows' partition and
+ * join milliseconds, oldest first (synchronises the ctx).  With
+ * SJB_F_OVERLAP the phases of consecutive windows may overlap in time.     */
 int sjb_timing_ring(sjb_ctx_t *ctx, uint32_t n);
 int sjb_timing_read(sjb_ctx_t *ctx, float *part_ms, float *join_ms, uint32_t n, uint32_t *n_read);
 

@@ -207,3 +207,32 @@ def test_full_size_window(ctx, torch_mod, orc, sjbmod):
         st = run_count(ctx, torch_mod, alg, R, S)
         assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
     assert 0.9 * 2**20 < o.count < 1.1 * 2**20
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_overlap_async_equals_plain(sjbmod, torch_mod, orc, alg):
+    # back-to-back async windows with the partition pass of window w+1 on a
+    # second stream (SJB_F_OVERLAP) give the same per-window results; a
+    # plain call in between and the timing ring keep working
+    W, nw = 1 << 16, 9
+    R = sjbmod.generate(W * nw, sjbmod.DIST_UNIFORM, 1 << 15, 0.0, seed=61)
+    S = sjbmod.generate(W * nw, sjbmod.DIST_UNIFORM, 1 << 15, 0.0, seed=62)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    with sjbmod.Context(max_window=W) as c:
+        c.timing_ring(2 * nw)
+        outs = []
+        for flags in (F_C2, F_C2 | sjbmod.F_OVERLAP | sjbmod.F_TIMING):
+            d = torch_mod.zeros(nw * 32, dtype=torch_mod.uint8, device="cuda")
+            for w in range(nw):
+                c.lazy_join_async(alg, flags, dR.data_ptr() + w * W * 16, W, dS.data_ptr() + w * W * 16, W,
+                                  d.data_ptr() + w * 32)
+                if w == 4:
+                    st = c.lazy_join(alg, 0, F_C2, dR, W, dS, W)  # plain call on buffer set 0
+            c.sync()
+            outs.append(d.cpu().numpy().view(sjbmod.WRES_DTYPE).copy())
+        assert np.array_equal(outs[0], outs[1])
+        part, join = c.timing_read(nw)
+        assert len(part) == nw and np.all(part > 0) and np.all(join > 0)
+    for w in (0, nw - 1):
+        o, _ = orc.lazy_hj(R[w * W:(w + 1) * W], S[w * W:(w + 1) * W], 1 << 12)
+        assert (int(outs[1][w]["count"]), int(outs[1][w]["checksum1"]), int(outs[1][w]["checksum2"])) == o.triple()
