@@ -42,7 +42,10 @@ constexpr int kHjNB = 1 << kHjLogNB;
 constexpr int kBatch = 8;
 constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
 constexpr int kPartialsPerPart = kJoinThreads / kWave;  // single-window partials per partition
-constexpr int kHjProbe = 2048;                          // probe records staged in LDS per pass
+#ifndef SJB_HJ_PROBE
+#define SJB_HJ_PROBE 2048
+#endif
+constexpr int kHjProbe = SJB_HJ_PROBE;                  // probe records staged in LDS per pass
 constexpr int kHjPerP = kHjProbe / kJoinThreads;        // staged probe records per thread
 
 // Record index of element j (< pref[nc]) by binary search over the run
@@ -497,7 +500,7 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
 // the table, every probe record is looked up against it and it is cleared,
 // so the table never exceeds 3/4 load.  Key 0xFFFFFFFF (the empty marker)
 // has its own slot.
-constexpr int kAggLog = 11;
+constexpr int kAggLog = kJoinThreads >= 1024 ? 12 : 11;
 constexpr int kAggSlots = 1 << kAggLog;
 constexpr uint32_t kAggHigh = kAggSlots / 2;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
