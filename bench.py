@@ -36,9 +36,9 @@ def parse():
     ap.add_argument("--log-stream", type=int, default=26)
     ap.add_argument("--key-range-log", type=int, default=24)
     ap.add_argument("--no-c2", action="store_true")
-    ap.add_argument("--no-overlap", action="store_true",
-                    help="window: run each window's partition and join on one stream (no overlap of "
-                         "window w+1's partition with window w's join)")
+    ap.add_argument("--overlap", action="store_true",
+                    help="window: run window w+1's partition pass on a second stream, concurrently with "
+                         "window w's join (+~3%% throughput; per-kernel durations then overlap)")
     ap.add_argument("--cpu-baseline-windows", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -334,7 +334,7 @@ def main():
     nw = N // W
     K = 1 << a.key_range_log
     alg = sjb.ALG[a.alg]
-    flags = (0 if a.no_c2 else sjb.F_C2) | sjb.F_TIMING | (0 if a.no_overlap else sjb.F_OVERLAP)
+    flags = (0 if a.no_c2 else sjb.F_C2) | sjb.F_TIMING | (sjb.F_OVERLAP if a.overlap else 0)
 
     t0 = time.time()
     R = sjb.generate(N, sjb.DIST_UNIFORM, K, 0.0, seed=123)
@@ -419,7 +419,7 @@ def main():
                    "windows_per_step": nw, "checksums": "count+C1" + ("" if a.no_c2 else "+C2"),
                    "parallelism": "single GPU",
                    "overlap": "partition of window w+1 concurrent with join of window w (2 streams)"
-                   if not a.no_overlap else "none"},
+                   if a.overlap else "none"},
         "device_time_s": dev_s,
         "window_ms": {"p50": float(np.percentile(win_ms, 50)), "p99": float(np.percentile(win_ms, 99)),
                       "mean": float(win_ms.mean())},
