@@ -190,7 +190,7 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, (1 << p.logF) * (kJoinThreads / kWave), d_res, p.peel ? c->rmi_nheavy : 0,
+    rc = launch_reduce(c, 1 << p.logF, d_res, p.peel ? c->rmi_nheavy : 0,
                        p.heavy);
     if (rc) return rc;
     if (ev) SJB_HIP_TRY(hipEventRecord(ev[3], c->stream));

@@ -41,7 +41,6 @@ namespace sjb {
 constexpr int kHjNB = 1 << kHjLogNB;
 constexpr int kBatch = 8;
 constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records per thread per tile
-constexpr int kPartialsPerPart = kJoinThreads / kWave;  // single-window partials per partition
 #ifndef SJB_HJ_PROBE
 #define SJB_HJ_PROBE 2048
 #endif
@@ -276,42 +275,41 @@ __device__ __forceinline__ void pair_acc(uint32_t k, uint32_t rv, uint32_t sv, A
     if (C2) acc.c2 += pair_hash_from_a(a, sv);
 }
 
-// Matches of one wave waiting for their checksum math: the index of the
-// probe record in the staging buffer and the build payload.  Queuing them
-// and running the 64-bit math 64 matches at a time keeps every lane busy,
-// where running it in place costs a full wave pass for each probe round in
-// which any lane matched.
-constexpr int kPairQ = 128;
+// Matches of one wave waiting for their checksum math: key, probe val and
+// build val.  Queuing them and running the 64-bit math up to 64 matches at
+// a time keeps the lanes busy, where running it in place costs a full wave
+// pass for each probe round in which any lane matched.
 struct PairQ {
-    uint32_t j[kPairQ];
-    uint32_t rv[kPairQ];
+    uint32_t key[kWave];
+    uint32_t sv[kWave];
+    uint32_t rv[kWave];
 };
 
-// Lanes [0, n) (n <= 64) each take one queued match.
+// Lanes [0, n) (n <= 64) each take one queued match; the queue is free
+// again afterwards.
 template <bool C2>
-__device__ __forceinline__ void pairq_drain(const PairQ &q, const Rec *sp, uint32_t n, Acc &acc) {
+__device__ __forceinline__ void pairq_drain(const PairQ &q, uint32_t n, Acc &acc) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (lane < n) {
-        const Rec s = sp[q.j[lane]];
-        pair_acc<C2>(s.x, q.rv[lane], s.y, acc);
-    }
+    if (lane < n) pair_acc<C2>(q.key[lane], q.rv[lane], q.sv[lane], acc);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
-// Count / C1 / C2 of the un probe records staged in LDS against the
-// current tile (payload = val).  The bucket scan compares keys only; the
-// first two matches of every probe go to the wave's queue (later matches,
-// rare, are summed in place).
+// Count / C1 / C2 of up to kHjPerP register-resident probe records per
+// thread (element j = k * blockDim + tid < un) against the current tile
+// (payload = val).  The bucket scan compares keys only; the first two
+// matches of every probe go to the wave's queue (later matches, rare, are
+// summed in place).
 template <bool C2>
-__device__ __forceinline__ void hj_probe_compact(const HjTable &t, const Rec *sp, uint32_t un, int bshift,
-                                                 Acc &acc, PairQ &q) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
+__device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[kHjPerP], uint32_t un,
+                                              int bshift, Acc &acc, PairQ &q) {
     uint32_t qn = 0;  // queued matches (wave-uniform)
-    for (uint32_t j0 = 0; j0 < un; j0 += kJoinThreads) {
-        const uint32_t j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kHjPerP; k++) {
+        const uint32_t j = k * kJoinThreads + threadIdx.x;
+        const Rec s = rp[k];
         uint32_t m = 0, rv0 = 0, rv1 = 0;
         if (j < un) {
-            const Rec s = sp[j];
             const uint32_t b = hj_bucket(s.x, bshift);
             const uint32_t lo = t.off[b], hi = t.off[b + 1];
             for (uint32_t e = lo; e < hi; e++) {
@@ -332,47 +330,33 @@ __device__ __forceinline__ void hj_probe_compact(const HjTable &t, const Rec *sp
             const bool has = m > r;
             const uint64_t mask = __ballot(has);
             if (mask == 0) break;  // wave-uniform
+            const uint32_t c = uint32_t(__popcll(mask));
+            if (qn + c > uint32_t(kWave)) {
+                pairq_drain<C2>(q, qn, acc);
+                qn = 0;
+            }
             if (has) {
                 const uint32_t pos = qn + lanes_below(mask);
-                q.j[pos] = j;
+                q.key[pos] = s.x;
+                q.sv[pos] = s.y;
                 q.rv[pos] = r == 0 ? rv0 : rv1;
             }
-            qn += uint32_t(__popcll(mask));
-            if (qn >= uint32_t(kWave)) {
-                pairq_drain<C2>(q, sp, kWave, acc);
-                // entries [64, qn) move down to [0, qn - 64)
-                uint32_t mj = 0, mr = 0;
-                const bool mv = lane < qn - kWave;
-                if (mv) {
-                    mj = q.j[kWave + lane];
-                    mr = q.rv[kWave + lane];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                if (mv) {
-                    q.j[lane] = mj;
-                    q.rv[lane] = mr;
-                }
-                qn -= kWave;
-            }
+            qn += c;
         }
     }
-    pairq_drain<C2>(q, sp, qn, acc);
+    pairq_drain<C2>(q, qn, acc);
 }
 
-// Probes the un records staged in LDS against the current tile.  WRITE
-// (payload = tuple index): per round one block scan of the lanes' match
-// counts and one global reservation, then rows are written.
-template <bool C2, bool WRITE>
-__device__ __forceinline__ void hj_probe_lds(const HjTable &t, const Rec *sp, uint32_t un, int bshift,
-                                             Acc &acc, const sjb_tuple_t *__restrict__ L,
-                                             const sjb_tuple_t *__restrict__ R,
-                                             sjb_result_tuple_t *__restrict__ out, uint64_t cap, int wrap,
-                                             unsigned long long *__restrict__ cursor, uint32_t *ws,
-                                             unsigned long long *s_base, PairQ &q) {
-    if (!WRITE) {
-        hj_probe_compact<C2>(t, sp, un, bshift, acc, q);
-        return;
-    }
+// Probes the un records staged in LDS against the current tile and writes
+// pair rows (payload = tuple index): per round one block scan of the lanes'
+// match counts and one global reservation, then rows are written.
+template <bool C2>
+__device__ __forceinline__ void hj_probe_write(const HjTable &t, const Rec *sp, uint32_t un, int bshift,
+                                               Acc &acc, const sjb_tuple_t *__restrict__ L,
+                                               const sjb_tuple_t *__restrict__ R,
+                                               sjb_result_tuple_t *__restrict__ out, uint64_t cap, int wrap,
+                                               unsigned long long *__restrict__ cursor, uint32_t *ws,
+                                               unsigned long long *s_base) {
     for (uint32_t j0 = 0; j0 < un; j0 += kJoinThreads) {
         const uint32_t j = j0 + threadIdx.x;
         Rec s = make_uint2(0, 0);
@@ -414,50 +398,59 @@ struct JoinIO {
     unsigned long long *cursor;
 };
 
-// LDS of the HJ body: the CSR build table and the probe staging buffer.
-struct HjLds {
+// LDS of the HJ body: the CSR build table and, when pair rows are written,
+// the probe staging buffer; otherwise probe records stay in registers and
+// each wave has a match queue.
+template <bool WRITE>
+struct HjLds;
+template <>
+struct HjLds<true> {
     HjTable t;
     Rec probe[kHjProbe];
+};
+template <>
+struct HjLds<false> {
+    HjTable t;
     PairQ q[kJoinThreads / kWave];
 };
 
 // Joins one partition (HJ).  The first build tile and the first kHjProbe
 // probe records are gathered with all loads in flight at once (build
-// records go straight into the LDS table, probe records into an LDS
-// staging buffer); larger partitions continue tile by tile.  The caller
-// has built `v` (load_view ends with a barrier).
+// records go straight into the LDS table; probe records stay in registers,
+// or, when pair rows are written, go to an LDS staging buffer); larger
+// partitions continue tile by tile.  The caller has built `v` (load_view
+// ends with a barrier).
 template <bool C2, bool WRITE>
 __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v, int bshift,
-                                             HjLds &m, uint32_t *s_ws, unsigned long long *s_base,
+                                             HjLds<WRITE> &m, uint32_t *s_ws, unsigned long long *s_base,
                                              Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
     if (!nb || !np) return;
     HjTable &t = m.t;
-    Rec *s_probe = m.probe;
     const Rec *__restrict__ part = io.part;
     for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
-    PairQ &q = m.q[threadIdx.x / kWave];
     const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjProbe);
     Rec rb[kHjPerT];
-    {
-        Rec rp[kHjPerP];
+    Rec rp[kHjPerP];
 #pragma unroll
-        for (int k = 0; k < kHjPerT; k++) {
-            const uint32_t i = k * kJoinThreads + threadIdx.x;
-            if (i < tn0) rb[k] = part[v.index(0, i)];
-        }
+    for (int k = 0; k < kHjPerT; k++) {
+        const uint32_t i = k * kJoinThreads + threadIdx.x;
+        if (i < tn0) rb[k] = part[v.index(0, i)];
+    }
+#pragma unroll
+    for (int k = 0; k < kHjPerP; k++) {
+        const uint32_t j = k * kJoinThreads + threadIdx.x;
+        rp[k] = make_uint2(0, 0);
+        if (j < pn0) rp[k] = part[v.index(1, j)];
+    }
+    if constexpr (WRITE) {
 #pragma unroll
         for (int k = 0; k < kHjPerP; k++) {
             const uint32_t j = k * kJoinThreads + threadIdx.x;
-            if (j < pn0) rp[k] = part[v.index(1, j)];
-        }
-#pragma unroll
-        for (int k = 0; k < kHjPerP; k++) {
-            const uint32_t j = k * kJoinThreads + threadIdx.x;
-            if (j < pn0) s_probe[j] = rp[k];
+            if (j < pn0) m.probe[j] = rp[k];
         }
     }
-    uint32_t resident = 0;  // first element of the probe chunk in s_probe
+    uint32_t resident = 0;  // first element of the resident probe chunk
     for (uint32_t t0 = 0; t0 < nb; t0 += kHjCap) {
         const uint32_t tn = min<uint32_t>(kHjCap, nb - t0);
         if (t0 > 0) {
@@ -472,14 +465,26 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
         hj_build(t, rb, tn, bshift, s_ws);
         for (uint32_t u0 = 0; u0 < np; u0 += kHjProbe) {
             const uint32_t un = min<uint32_t>(kHjProbe, np - u0);
-            if (u0 != resident) {
-                __syncthreads();
-                gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { s_probe[i] = r; });
-                resident = u0;
-                __syncthreads();
+            if constexpr (WRITE) {
+                if (u0 != resident) {
+                    __syncthreads();
+                    gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { m.probe[i] = r; });
+                    resident = u0;
+                    __syncthreads();
+                }
+                hj_probe_write<C2>(t, m.probe, un, bshift, acc, io.L, io.R, io.out, io.out_cap, io.wrap,
+                                   io.cursor, s_ws, s_base);
+            } else {
+                if (u0 != resident) {
+#pragma unroll
+                    for (int k = 0; k < kHjPerP; k++) {
+                        const uint32_t j = k * kJoinThreads + threadIdx.x;
+                        if (j < un) rp[k] = part[v.index(1, u0 + j)];
+                    }
+                    resident = u0;
+                }
+                hj_probe_regs<C2>(t, rp, un, bshift, acc, m.q[threadIdx.x / kWave]);
             }
-            hj_probe_lds<C2, WRITE>(t, s_probe, un, bshift, acc, io.L, io.R, io.out, io.out_cap,
-                                    io.wrap, io.cursor, s_ws, s_base, q);
         }
         __syncthreads();
     }
@@ -855,9 +860,9 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
 }
 
 // LDS bytes of each join body.
-template <int ALG, bool GROUPED>
+template <int ALG, bool GROUPED, bool WRITE>
 struct JoinLdsBytes {
-    static constexpr size_t hj = GROUPED ? sizeof(AggLds) : sizeof(HjLds);
+    static constexpr size_t hj = GROUPED ? sizeof(AggLds) : sizeof(HjLds<WRITE>);
     static constexpr size_t smj = GROUPED ? (sizeof(SmjLds) > sizeof(AggLds) ? sizeof(SmjLds) : sizeof(AggLds))
                                           : sizeof(SmjLds);
     static constexpr size_t value = ALG == SJB_ALG_SMJ ? smj : hj;
@@ -871,7 +876,7 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
         if constexpr (GROUPED)
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            hj_join_part<C2, WRITE>(io, v, bshift, *reinterpret_cast<HjLds *>(mem), s_ws, s_base, acc);
+            hj_join_part<C2, WRITE>(io, v, bshift, *reinterpret_cast<HjLds<WRITE> *>(mem), s_ws, s_base, acc);
     } else {
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
@@ -888,14 +893,15 @@ __host__ __device__ constexpr int idcap_of() {
 // ---------------------------------------------------------------------------
 // one window: one workgroup per partition
 // ---------------------------------------------------------------------------
-// Per-wave partials go to partials[(p * kPartialsPerPart + wave) * 3 ...]
-// and are summed by k_reduce.
+// The workgroup's (count, C1, C2) goes to partials[3p ...] and is summed
+// by k_reduce.
 template <int ALG, bool C2, bool WRITE, bool GROUPED>
 __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t *__restrict__ offs,
                                                        int logF, int ncL, int ncR, uint32_t nL,
                                                        int bshift, uint64_t *__restrict__ partials) {
-    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED>::value];
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, WRITE>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
+    __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ unsigned long long s_base;
     extern __shared__ uint32_t dyn[];
 
@@ -904,15 +910,11 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
-    // one partial per wave (summed by k_reduce): no block-level reduction
-    acc.count = wave_sum(acc.count);
-    acc.c1 = wave_sum(acc.c1);
-    acc.c2 = wave_sum(acc.c2);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const uint64_t w = uint64_t(p) * kPartialsPerPart + threadIdx.x / kWave;
-        partials[3 * w] = acc.count;
-        partials[3 * w + 1] = acc.c1;
-        partials[3 * w + 2] = acc.c2;
+    const Acc r = block_sum(acc, s_red);
+    if (threadIdx.x == 0) {
+        partials[3 * p] = r.count;
+        partials[3 * p + 1] = r.c1;
+        partials[3 * p + 2] = r.c2;
     }
 }
 
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
 // the same bits).
 template <int ALG, bool C2, bool GROUPED>
 __global__ __launch_bounds__(kJoinThreads) void k_join_batch(BatchArgs a) {
-    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED>::value];
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, false>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ unsigned long long s_base;

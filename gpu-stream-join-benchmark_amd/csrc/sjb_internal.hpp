@@ -35,7 +35,7 @@ typedef uint2 Rec;
 #define SJB_JOIN_THREADS 512
 #endif
 #ifndef SJB_HJ_CAP
-#define SJB_HJ_CAP 3072
+#define SJB_HJ_CAP 2560
 #endif
 #ifndef SJB_HJ_LOGNB
 #define SJB_HJ_LOGNB 11
