@@ -303,6 +303,184 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
                                              heavy_acc);
 }
 
+// ---------------------------------------------------------------------------
+// persistent, software-pipelined variant of k_partition (no heavy peeling)
+// ---------------------------------------------------------------------------
+// The staging buffer holds a whole chunk (128 KiB), so only one partition
+// workgroup fits on a CU and, one chunk per workgroup, a CU would sit idle on
+// HBM while it ranks, scans and stages.  Here each workgroup walks the chunks
+// b = blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of chunk
+// b + gridDim.x as soon as chunk b's keys are ranked: they are in flight
+// while chunk b is scanned, staged and written back.  Same records, offsets
+// and layout as k_partition.
+constexpr int kPartPer = kChunk / kPartThreads;  // tuples per thread per chunk
+
+struct ChunkGeo {
+    const sjb_tuple_t *in;
+    uint64_t base;  // first tuple of the chunk within its side
+    uint64_t obase; // first record of the chunk in the partition output
+    int len;
+};
+
+__device__ __forceinline__ ChunkGeo chunk_geo(int b, const sjb_tuple_t *L, uint64_t nL, int ncL,
+                                              const sjb_tuple_t *R, uint64_t nR) {
+    ChunkGeo g;
+    const bool left = b < ncL;
+    const int c = left ? b : b - ncL;
+    const uint64_t n = left ? nL : nR;
+    g.in = left ? L : R;
+    g.base = uint64_t(c) * kChunk;
+    g.obase = (left ? 0 : nL) + g.base;
+    g.len = int(min<uint64_t>(kChunk, n - g.base));
+    return g;
+}
+
+// Loads are clamped to the chunk's last tuple instead of guarded, so they
+// are issued back to back without branches; lanes past len are ignored.
+// Addresses are a uniform base plus a 32-bit lane offset (one VGPR each).
+// Loads records [J0, J1) of every thread.
+template <int PAY, int J0 = 0, int J1 = kPartPer>
+__device__ __forceinline__ void chunk_load(const ChunkGeo &g, Rec (&rec)[kPartPer]) {
+    const char *__restrict__ in = reinterpret_cast<const char *>(g.in + g.base);
+    const uint32_t last = uint32_t(g.len - 1);
+#pragma unroll
+    for (int j = J0; j < J1; j++) {
+        const uint32_t e = min(uint32_t(j * kPartThreads) + threadIdx.x, last);
+        const uint2 kv = *reinterpret_cast<const uint2 *>(in + e * uint32_t(sizeof(sjb_tuple_t)));
+        rec[j] = make_uint2(kv.x, PAY == PAY_VAL ? kv.y : uint32_t(g.base) + e);
+    }
+}
+
+// rank[j/2] holds the ranks of records j (low half) and j+1 (high half)
+// within their partitions (< CHUNK = 2^14); the partition is recomputed
+// when the record is staged.  The partition function runs on every
+// (clamped, valid) record outside the len guard, so every record register
+// is consumed unconditionally here.
+template <int KIND>
+__device__ __forceinline__ void chunk_rank(const PartFn<KIND> &part, const Rec (&rec)[kPartPer], int len,
+                                           uint32_t *hist, uint32_t (&rank)[kPartPer / 2]) {
+#pragma unroll
+    for (int j = 0; j < kPartPer; j++) {
+        const int e = j * kPartThreads + threadIdx.x;
+        const uint32_t p = part(rec[j].x);
+        uint32_t r = 0;
+        if (e < len) r = atomicAdd(&hist[p], 1u);
+        if (j & 1)
+            rank[j / 2] |= r << 16;
+        else
+            rank[j / 2] = r;
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ void chunk_stage(const PartFn<KIND> &part, const Rec (&rec)[kPartPer],
+                                            const uint32_t (&rank)[kPartPer / 2], int len,
+                                            const uint32_t *hist, Rec *stage) {
+#pragma unroll
+    for (int j = 0; j < kPartPer; j++) {
+        const int e = j * kPartThreads + threadIdx.x;
+        const uint32_t r = (j & 1) ? rank[j / 2] >> 16 : rank[j / 2] & 0xFFFFu;
+        if (e < len) stage[hist[part(rec[j].x)] + r] = rec[j];
+    }
+}
+
+// Exclusive scan of hist[0, F) in place, hist[F] = total; every thread of
+// the block calls it; ends with a barrier.
+__device__ __forceinline__ void hist_scan(uint32_t *hist, uint32_t *ws, int F) {
+    constexpr int kMaxPer = ((1 << kMaxLogF) + kPartThreads - 1) / kPartThreads;
+    uint32_t v[kMaxPer];
+    uint32_t sum = 0;
+    const int per = (F + kPartThreads - 1) / kPartThreads;
+    for (int q = 0; q < per && q < kMaxPer; q++) {
+        const int idx = threadIdx.x * per + q;
+        v[q] = idx < F ? hist[idx] : 0;
+        sum += v[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan_u32(sum, ws, &tot);
+    for (int q = 0; q < per && q < kMaxPer; q++) {
+        const int idx = threadIdx.x * per + q;
+        if (idx < F) hist[idx] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) hist[F] = tot;
+    __syncthreads();
+}
+
+static_assert(kChunk <= (1 << 16), "chunk_rank packs ranks into 16 bits");
+
+// One chunk b (records in `cur`, geometry g): rank, scan, stage, write the
+// offsets row and the records.  The next chunk's loads (gn, into `nxt`),
+// when there is one, are issued in two halves: the first right after the
+// ranking, the second once `cur` is staged (its registers are then free),
+// so at most one and a half chunks occupy registers.  Starts and ends with
+// the LDS free.
+template <int KIND, int PAY>
+__device__ __forceinline__ void chunk_step(const PartFn<KIND> &part, int F, uint32_t *hist, uint32_t *ws,
+                                           Rec *stage, const Rec (&cur)[kPartPer], const ChunkGeo &g, int b,
+                                           Rec (&nxt)[kPartPer], const ChunkGeo &gn, bool has_next,
+                                           uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    for (int q = threadIdx.x; q <= F; q += kPartThreads) hist[q] = 0;
+    __syncthreads();
+    uint32_t rank[kPartPer / 2];
+    chunk_rank<KIND>(part, cur, g.len, hist, rank);
+    if (has_next) chunk_load<PAY, 0, kPartPer / 2>(gn, nxt);
+    __syncthreads();
+    hist_scan(hist, ws, F);
+    chunk_stage<KIND>(part, cur, rank, g.len, hist, stage);
+    if (has_next) chunk_load<PAY, kPartPer / 2, kPartPer>(gn, nxt);
+    // unsigned 32-bit lane offsets from uniform bases: the stores address
+    // through a scalar base, with no per-lane 64-bit pointers kept live
+    // The lane index is re-read here (the empty asm hides its value) so the
+    // lane addresses below are formed per step rather than hoisted out of
+    // the chunk loop and kept live (or spilled) across it.
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    uint32_t *o = offs + uint32_t(b) * uint32_t(F + 1);  // < 2^23 words
+    for (uint32_t q = tid; q <= uint32_t(F); q += kPartThreads) o[q] = hist[q];
+    __syncthreads();
+    Rec *out = part_out + g.obase;
+    for (uint32_t e = tid; e < uint32_t(g.len); e += kPartThreads) out[e] = stage[e];
+    __syncthreads();
+}
+
+// Two register sets alternate (no copies between them), so waiting for one
+// chunk's loads never waits for the previous chunk's stores.
+template <int KIND, int PAY>
+__global__ __launch_bounds__(kPartThreads) void k_partition_pipe(
+    const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
+    uint64_t nR, int ncR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    extern __shared__ uint32_t lds[];
+    const int F = 1 << logF;
+    uint32_t *hist = lds;       // F + 1 counters
+    uint32_t *ws = lds + F + 2; // wave sums
+    Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 2 + kPartThreads / kWave + 1) & ~1));
+    PartFn<KIND> part;
+    part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
+    const int nch = ncL + ncR;
+    const int G = int(gridDim.x);
+    int b = blockIdx.x;
+    if (b >= nch) return;
+    ChunkGeo ga = chunk_geo(b, L, nL, ncL, R, nR);
+    Rec ra[kPartPer], rb[kPartPer];
+    chunk_load<PAY>(ga, ra);
+    for (;;) {
+        const int b1 = b + G;
+        const bool h1 = b1 < nch;
+        const ChunkGeo gb = h1 ? chunk_geo(b1, L, nL, ncL, R, nR) : ga;
+        chunk_step<KIND, PAY>(part, F, hist, ws, stage, ra, ga, b, rb, gb, h1, offs, part_out);
+        if (!h1) break;
+        const int b2 = b1 + G;
+        const bool h2 = b2 < nch;
+        const ChunkGeo gc = h2 ? chunk_geo(b2, L, nL, ncL, R, nR) : gb;
+        chunk_step<KIND, PAY>(part, F, hist, ws, stage, rb, gb, b1, ra, gc, h2, offs, part_out);
+        if (!h2) break;
+        b = b2;
+        ga = gc;
+    }
+}
+
 // Range parameters of a batched window from a fixed strided sample of up
 // to 256 keys per side.  Every chunk workgroup of the window draws the same
 // sample, so all of them use the same partition function; keys outside the
@@ -446,6 +624,17 @@ static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
                           const sjb_tuple_t *R) {
     const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;
+#if SJB_PART_PIPE
+    if constexpr (!PEEL && KIND != PART_LEARNED) {
+        if (p.payload == PAY_VAL) {
+            const int grid = std::min(nblk, std::max(c->n_cu, 1));
+            k_partition_pipe<KIND, PAY_VAL><<<grid, kPartThreads, lds, p.ps>>>(
+                L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part);
+            SJB_HIP_TRY(hipGetLastError());
+            return SJB_OK;
+        }
+    }
+#endif
     if (p.payload == PAY_VAL)
         k_partition<KIND, PAY_VAL, PEEL><<<nblk, kPartThreads, lds, p.ps>>>(
             L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy);
@@ -569,6 +758,8 @@ int partition_setup(sjb_ctx *c) {
         (const void *)k_partition_batch<PART_RANGE, 4096, false>,
         (const void *)k_partition_batch<PART_LEARNED, 4096, false>,
         (const void *)k_partition_batch<PART_LEARNED, 4096, true>,
+        (const void *)k_partition_pipe<PART_HASH, PAY_VAL>,
+        (const void *)k_partition_pipe<PART_RANGE, PAY_VAL>,
     };
     for (const void *f : fns)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, need);

@@ -31,6 +31,9 @@ typedef uint2 Rec;
 #ifndef SJB_PART_THREADS
 #define SJB_PART_THREADS 1024
 #endif
+#ifndef SJB_PART_PIPE
+#define SJB_PART_PIPE 1
+#endif
 #ifndef SJB_JOIN_THREADS
 #define SJB_JOIN_THREADS 512
 #endif
