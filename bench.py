@@ -351,13 +351,13 @@ def main():
     base_r, base_s, base_res = dR.data_ptr(), dS.data_ptr(), d_res.data_ptr()
 
     def step(evs=None):
+        if evs is not None:
+            evs[0].record(stream)
         for w in range(nw):
-            if evs is not None:
-                evs[w].record(stream)
             ctx.lazy_join_async(alg, flags, base_r + w * W * 16, W, base_s + w * W * 16, W,
                                 base_res + w * 32)
         if evs is not None:
-            evs[nw].record(stream)
+            evs[1].record(stream)
 
     for _ in range(a.warmup):
         step()
@@ -365,7 +365,9 @@ def main():
     torch.cuda.synchronize()
     first = d_res.cpu().numpy().view(sjb.WRES_DTYPE).copy()
 
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(nw + 1)] for _ in range(a.steps)]
+    # step boundaries only; per-window and per-phase times come from the
+    # library's timestamp ring (4 fence-free events per window)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(a.steps)]
     torch.cuda.synchronize()
     ctx.sync()
     t_start = time.perf_counter()
@@ -375,9 +377,9 @@ def main():
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_start
 
-    win_ms = np.array([evs[s][w].elapsed_time(evs[s][w + 1]) for s in range(a.steps) for w in range(nw)])
-    dev_s = sum(evs[s][0].elapsed_time(evs[s][nw]) for s in range(a.steps)) / 1e3
+    dev_s = sum(evs[s][0].elapsed_time(evs[s][1]) for s in range(a.steps)) / 1e3
     part_ms, join_ms = ctx.timing_read(nw * a.steps)  # library events around each phase
+    win_ms = part_ms + join_ms  # device time of each window's kernels
     last = d_res.cpu().numpy().view(sjb.WRES_DTYPE)
     if a.warmup > 0 and not np.array_equal(first, last):
         raise SystemExit("window results changed between steps")

@@ -238,11 +238,12 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     a.qhead = p->d_qhead;
     a.chunk = chunk;
     a.grab = uint32_t(std::max(p->tune.grab, 1));
-    a.idcap = uint32_t(join_idcap(p->alg));
+    const bool grouped = (p->flags & SJB_F_GROUPED) != 0;
+    a.idcap = uint32_t(join_idcap(p->alg, grouped));
     a.rmi = c->d_rmi;
     a.heavy = p->peel ? p->d_heavy : nullptr;
     p->dyn_part = partition_lds_bytes_chunk(max_logF, int(chunk));
-    p->dyn_join = join_view_bytes(p->alg, max_ncL, max_ncR);
+    p->dyn_join = join_view_bytes(p->alg, grouped, max_ncL, max_ncR);
     p->blocks = int(std::min<uint64_t>(uint64_t(c->n_cu) * std::max(p->tune.blocks_per_cu, 1),
                                        std::max<uint32_t>(n_items, 1)));
     return SJB_OK;
@@ -431,7 +432,7 @@ int sjb_batch_set_tune(sjb_ctx_t *c, const sjb_batch_tune_t *t) {
     if (!c || !t) return SJB_E_INVALID;
     if (t->fanout_bias < -4 || t->fanout_bias > 4) return SJB_E_INVALID;
     if (t->chunk_log2 != 12 && t->chunk_log2 != 14) return SJB_E_INVALID;
-    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 2 || t->grab < 1 || t->grab > 64) return SJB_E_INVALID;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 3 || t->grab < 1 || t->grab > 64) return SJB_E_INVALID;
     c->tune.fanout_bias = t->fanout_bias;
     c->tune.chunk_log2 = int(t->chunk_log2);
     c->tune.blocks_per_cu = int(t->blocks_per_cu);
@@ -443,7 +444,7 @@ int sjb_batch_set_tune(sjb_ctx_t *c, const sjb_batch_tune_t *t) {
 // run, then `reps` timed runs, median), checks that every candidate gives
 // bit-identical per-window results, keeps the fastest in the ctx and
 // reports it.  Candidates: fanout bias {-1,0,+1} x chunk {2^12,2^14} x
-// join workgroups per CU {1,2} x queue grab {1,4}.
+// join workgroups per CU {1,2,3} x queue grab {1,4}.
 int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t *d_left,
                        const uint64_t *left_off, const sjb_tuple_t *d_right,
                        const uint64_t *right_off, uint32_t n_windows, uint32_t reps,
@@ -459,7 +460,7 @@ int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t 
     uint32_t tried = 0;
     for (int bias = -1; bias <= 1; bias++)
         for (int cl : {12, 14})
-            for (int bpc : {1, 2})
+            for (int bpc : {1, 2, 3})
                 for (int grab : {1, 4}) {
                     BatchTune t;
                     t.fanout_bias = bias;

@@ -45,6 +45,9 @@ constexpr int kHjPerT = (kHjCap + kJoinThreads - 1) / kJoinThreads;  // records 
 #define SJB_HJ_PROBE 2048
 #endif
 constexpr int kHjProbe = SJB_HJ_PROBE;                  // probe records staged in LDS per pass
+#ifndef SJB_BATCH_HJ_WAVES
+#define SJB_BATCH_HJ_WAVES 6
+#endif
 constexpr int kHjPerP = kHjProbe / kJoinThreads;        // staged probe records per thread
 
 // Record index of element j (< pref[nc]) by binary search over the run
@@ -509,7 +512,7 @@ constexpr int kAggLog = kJoinThreads >= 1024 ? 12 : 11;
 constexpr int kAggSlots = 1 << kAggLog;
 constexpr uint32_t kAggHigh = kAggSlots / 2;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-constexpr int kAggProbe = 2048;  // probe records staged in LDS per pass
+constexpr int kAggIdCap = 1536;  // run-id table entries per side of a grouped HJ view
 static_assert(kAggHigh + kJoinThreads <= (kAggSlots * 3) / 4, "aggregation table load");
 
 struct AggLds {
@@ -520,7 +523,6 @@ struct AggLds {
     uint64_t ff_suma;           // key 0xFFFFFFFF
     uint32_t ff_cnt, ff_fval;
     uint32_t distinct;
-    Rec probe[kAggProbe];
 };
 
 __device__ __forceinline__ uint32_t agg_slot(uint32_t k, int bshift) {
@@ -627,22 +629,23 @@ __device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &
     const uint32_t nb = v.n[0], np = v.n[1];
     if (!nb || !np) return;
     const Rec *__restrict__ part = io.part;
-    const uint32_t pn0 = min<uint32_t>(np, kAggProbe);
     agg_clear(t);
-    gather(part, v, 1, 0, pn0, [&](uint32_t i, Rec r) { t.probe[i] = r; });
     __syncthreads();
-    uint32_t resident = 0;
-    // probe every probe record against the table
+    // probe every probe record against the table (read-only meanwhile):
+    // kBatch register-resident records per thread per pass
     auto probe_all = [&]() {
-        for (uint32_t u0 = 0; u0 < np; u0 += kAggProbe) {
-            const uint32_t un = min<uint32_t>(kAggProbe, np - u0);
-            if (u0 != resident) {
-                __syncthreads();
-                gather(part, v, 1, u0, un, [&](uint32_t i, Rec r) { t.probe[i] = r; });
-                resident = u0;
-                __syncthreads();
+        for (uint32_t u0 = 0; u0 < np; u0 += kJoinThreads * kBatch) {
+            Rec r[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; k++) {
+                const uint32_t j = u0 + k * kJoinThreads + threadIdx.x;
+                if (j < np) r[k] = part[v.index(1, j)];
             }
-            for (uint32_t j = threadIdx.x; j < un; j += kJoinThreads) agg_probe(t, t.probe[j], bshift, acc);
+#pragma unroll
+            for (int k = 0; k < kBatch; k++) {
+                const uint32_t j = u0 + k * kJoinThreads + threadIdx.x;
+                if (j < np) agg_probe(t, r[k], bshift, acc);
+            }
         }
     };
     for (uint32_t i0 = 0; i0 < nb; i0 += kJoinThreads * kBatch) {
@@ -885,9 +888,12 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
     }
 }
 
-template <int ALG>
+// Run-id table entries per side of a view (records past them are located
+// by binary search): one build tile; grouped HJ keeps fewer so its
+// workgroup fits three to a CU.
+template <int ALG, bool GROUPED>
 __host__ __device__ constexpr int idcap_of() {
-    return ALG == SJB_ALG_SMJ ? kSmjCap : kHjCap;
+    return ALG == SJB_ALG_SMJ ? kSmjCap : (GROUPED ? kAggIdCap : kHjCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -906,7 +912,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
     extern __shared__ uint32_t dyn[];
 
     const int p = xcd_partition(blockIdx.x, 1 << logF);
-    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG>(), s_ws);
+    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws);
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
@@ -931,8 +937,12 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
 // the grid always drains.  Each item adds its (count, C1, C2) to its
 // window's result with three 64-bit atomics (sums mod 2^64: any order gives
 // the same bits).
+// HJ bodies fit three workgroups per CU in LDS; the register budget is
+// capped to match (6 waves per SIMD -> at most 80 VGPRs).
 template <int ALG, bool C2, bool GROUPED>
-__global__ __launch_bounds__(kJoinThreads) void k_join_batch(BatchArgs a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kJoinThreads),
+                          amdgpu_waves_per_eu(ALG == SJB_ALG_HJ ? SJB_BATCH_HJ_WAVES : 4))) void
+k_join_batch(BatchArgs a) {
     __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, false>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
@@ -956,7 +966,9 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_batch(BatchArgs a) {
             __syncthreads();
             if (threadIdx.x == 0) s_item = qlen ? atomicAdd(&a.qhead[x * 16], a.grab) : qlen;
             __syncthreads();
-            const uint32_t i0 = s_item;
+            // uniform: readfirstlane keeps the item and its window
+            // descriptor in scalar registers
+            const uint32_t i0 = __builtin_amdgcn_readfirstlane(s_item);
             if (i0 >= qlen) break;
             const uint32_t i1 = min(qlen, i0 + a.grab);
             for (uint32_t i = i0; i < i1; i++) {
@@ -1035,9 +1047,13 @@ int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32
     return SJB_OK;
 }
 
-int join_idcap(int alg) { return alg == SJB_ALG_SMJ ? kSmjCap : kHjCap; }
+int join_idcap(int alg, bool grouped) {
+    return alg == SJB_ALG_SMJ ? kSmjCap : (grouped ? kAggIdCap : kHjCap);
+}
 
-size_t join_view_bytes(int alg, int ncL, int ncR) { return part_view_bytes(ncL, ncR, join_idcap(alg)); }
+size_t join_view_bytes(int alg, bool grouped, int ncL, int ncR) {
+    return part_view_bytes(ncL, ncR, join_idcap(alg, grouped));
+}
 
 template <int ALG>
 static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R,
@@ -1046,7 +1062,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     const bool c2 = (p.flags & SJB_F_C2) != 0;
     const bool grouped = (p.flags & SJB_F_GROUPED) != 0 && p.payload == PAY_VAL;
     const int bshift = p.kind == PART_HASH ? p.logF : 0;
-    const size_t dynb = part_view_bytes(p.ncL, p.ncR, idcap_of<ALG>());
+    const size_t dynb = part_view_bytes(p.ncL, p.ncR, grouped ? idcap_of<ALG, true>() : idcap_of<ALG, false>());
     JoinIO io;
     io.part = p.part;
     io.L = L;

@@ -41,36 +41,72 @@ __device__ __forceinline__ uint32_t rmi_partition(const RmiParams *__restrict__ 
     return uint32_t(c);
 }
 
-// Heavy-key set of a learned model in LDS (64-slot open addressing):
-// slot -> key, index in RmiParams::heavy (-1 = empty).
+// Heavy-key set of a learned model in LDS (the RmiParams lookup table:
+// 64-slot open addressing, slot -> key, index in RmiParams::heavy, -1 =
+// empty, plus a 4096-bit filter on 12 hash bits, so most keys, not heavy,
+// are rejected with one LDS read instead of a probe sequence).
+// Each wave also has a queue of its heavy hits (key, val, heavy index):
+// hits are hashed and summed when 64 are queued, with every lane busy.
+struct HeavyQ {
+    uint32_t key[kWave];
+    uint32_t val[kWave];
+    int h[kWave];
+};
 struct HeavySet {
     uint32_t key[64];
     int idx[64];
+    uint32_t filter[128];
     uint32_t cnt[kMaxHeavy];
     uint64_t sum[kMaxHeavy];
+    HeavyQ q[kPartThreads / kWave];
 };
 
-// Thread 0 fills the set; the caller puts a barrier after it.
-__device__ __forceinline__ void heavy_set_load(HeavySet &h, const RmiParams *__restrict__ m) {
-    if (threadIdx.x < 64) h.idx[threadIdx.x] = -1;
-    if (threadIdx.x < kMaxHeavy) {
-        h.cnt[threadIdx.x] = 0;
-        h.sum[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t i = 0; i < m->n_heavy && i < uint32_t(kMaxHeavy); i++) {
-            const uint32_t k = m->heavy[i];
-            uint32_t s = (k * kHashMul) >> 26;
-            while (h.idx[s] >= 0 && h.key[s] != k) s = (s + 1) & 63;
-            h.key[s] = k;
-            h.idx[s] = int(i);
+constexpr int kHeavyReg = 4;  // most frequent heavy keys summed in registers
+
+// Hashes and sums the n (<= 64) queued heavy hits of the calling wave, one
+// per lane: the kHeavyReg most frequent keys into the lane's registers, the
+// others with LDS atomics.  The queue is free again afterwards.
+__device__ __forceinline__ void heavy_drain(HeavySet &hs, const HeavyQ &q, uint32_t n, bool left,
+                                            uint32_t (&hcnt)[kHeavyReg], uint64_t (&hsum)[kHeavyReg]) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane < n) {
+        const uint32_t k = q.key[lane], v = q.val[lane];
+        const int h = q.h[lane];
+        const uint64_t x = left ? hash_a(k, v) : hash_b(k, v);
+        if (h < kHeavyReg) {
+#pragma unroll
+            for (int i = 0; i < kHeavyReg; i++) {
+                hcnt[i] += h == i;
+                hsum[i] += h == i ? x : 0ull;
+            }
+        } else {
+            atomicAdd(&hs.cnt[h], 1u);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&hs.sum[h]), (unsigned long long)x);
         }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// Copies the model's precomputed lookup table (one word per thread) and
+// clears the counters; the caller puts a barrier after it.
+__device__ __forceinline__ void heavy_set_load(HeavySet &h, const RmiParams *__restrict__ m) {
+    const uint32_t t = threadIdx.x;
+    if (t < 64) {
+        h.key[t] = m->hslot_key[t];
+        h.idx[t] = m->hslot_idx[t];
+    }
+    if (t < 128) h.filter[t] = m->hfilter[t];
+    if (t < kMaxHeavy) {
+        h.cnt[t] = 0;
+        h.sum[t] = 0;
     }
 }
 
 __device__ __forceinline__ int heavy_find(const HeavySet &h, uint32_t k) {
-    uint32_t s = (k * kHashMul) >> 26;
+    const uint32_t x = k * kHashMul;
+    if (!((h.filter[x >> 25] >> ((x >> 20) & 31)) & 1u)) return -1;
+    uint32_t s = x >> 26;
     for (;;) {
         const int i = h.idx[s];
         if (i < 0) return -1;
@@ -190,10 +226,13 @@ int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, 
 // [o_c[p], o_c[p+1]) of every chunk c.
 //
 // PEEL (learned model, grouped mode): records whose key is in the model's
-// heavy set go to an extra bucket F after the last partition — they are
-// not written back (o[F] ends partition F-1) — and are aggregated instead:
-// per heavy key the count and the sum of A (left side) or B (right side)
-// are added to heavy_acc[4*h + (left ? 0 : 2) + {0,1}].
+// heavy set are neither ranked nor written back (o[F] ends partition F-1)
+// but aggregated: per heavy key the count and the sum of A (left side) or
+// B (right side) are added to heavy_acc[4*h + (left ? 0 : 2) + {0,1}].
+// Hits are queued per wave and hashed 64 at a time (heavy_drain); heavy
+// keys are ordered by sample frequency and the kHeavyReg most frequent are
+// summed in per-thread registers, reduced once per wave (under skew they
+// are most of the hits, and LDS atomics on one address serialise).
 template <int KIND, int PAY, int CHUNK, bool PEEL>
 __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ in, uint64_t base,
                                                 int len, int logF, const PartFn<KIND> &part,
@@ -211,6 +250,13 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
     constexpr int kPer = CHUNK / kPartThreads;
     Rec rec[kPer];
     uint32_t pid[kPer], rank[kPer];
+    uint32_t hcnt[kHeavyReg];
+    uint64_t hsum[kHeavyReg];
+#pragma unroll
+    for (int i = 0; i < kHeavyReg; i++) {
+        hcnt[i] = 0;
+        hsum[i] = 0;
+    }
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
         const int e = j * kPartThreads + threadIdx.x;
@@ -219,28 +265,52 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
             rec[j] = make_uint2(kv.x, PAY == PAY_VAL ? kv.y : uint32_t(base + e));
         }
     }
+    uint32_t qn = 0;  // queued heavy hits of this wave (wave-uniform)
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
         const int e = j * kPartThreads + threadIdx.x;
-        if (e < len) {
-            if (PEEL) {
-                const int h = heavy_find(*hs, rec[j].x);
-                if (h >= 0) {
-                    pid[j] = uint32_t(F);
-                    atomicAdd(&hs->cnt[h], 1u);
-                    const uint64_t v = left ? hash_a(rec[j].x, rec[j].y) : hash_b(rec[j].x, rec[j].y);
-                    atomicAdd(reinterpret_cast<unsigned long long *>(&hs->sum[h]), (unsigned long long)v);
-                } else {
-                    pid[j] = part(rec[j].x);
+        if (PEEL) {
+            const int h = e < len ? heavy_find(*hs, rec[j].x) : -1;
+            const bool hv = h >= 0;
+            const uint64_t mask = __ballot(hv);
+            if (mask) {  // wave-uniform
+                HeavyQ &q = hs->q[threadIdx.x / kWave];
+                const uint32_t c = uint32_t(__popcll(mask));
+                if (qn + c > uint32_t(kWave)) {
+                    heavy_drain(*hs, q, qn, left, hcnt, hsum);
+                    qn = 0;
                 }
-            } else {
-                pid[j] = part(rec[j].x);
+                if (hv) {
+                    const uint32_t pos = qn + lanes_below(mask);
+                    q.key[pos] = rec[j].x;
+                    q.val[pos] = rec[j].y;
+                    q.h[pos] = h;
+                }
+                qn += c;
             }
+            if (e < len) {
+                pid[j] = hv ? uint32_t(F) : part(rec[j].x);
+                if (!hv) rank[j] = atomicAdd(&hist[pid[j]], 1u);
+            }
+        } else if (e < len) {
+            pid[j] = part(rec[j].x);
             rank[j] = atomicAdd(&hist[pid[j]], 1u);
         }
     }
+    if (PEEL) {
+        heavy_drain(*hs, hs->q[threadIdx.x / kWave], qn, left, hcnt, hsum);
+#pragma unroll
+        for (int i = 0; i < kHeavyReg; i++) {
+            const uint64_t c = wave_sum(uint64_t(hcnt[i]));
+            const uint64_t v = wave_sum(hsum[i]);
+            if ((threadIdx.x & (kWave - 1)) == 0 && c) {
+                atomicAdd(&hs->cnt[i], uint32_t(c));
+                atomicAdd(reinterpret_cast<unsigned long long *>(&hs->sum[i]), (unsigned long long)v);
+            }
+        }
+    }
     __syncthreads();
-    // exclusive scan of hist[0..NB) in place; hist[NB] = len
+    // exclusive scan of hist[0..NB) in place; hist[NB] = records ranked
     {
         constexpr int kMaxPer = ((1 << kMaxLogF) + 1 + kPartThreads - 1) / kPartThreads;
         uint32_t v[kMaxPer];
@@ -265,7 +335,7 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
         const int e = j * kPartThreads + threadIdx.x;
-        if (e < len) stage[hist[pid[j]] + rank[j]] = rec[j];
+        if (e < len && (!PEEL || pid[j] < uint32_t(F))) stage[hist[pid[j]] + rank[j]] = rec[j];
     }
     __syncthreads();
     const int keep = PEEL ? int(hist[F]) : len;

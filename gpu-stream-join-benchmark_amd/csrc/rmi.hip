@@ -363,6 +363,24 @@ static double imbalance(const std::vector<uint32_t> &hist) {
     return tot ? double(mx) * double(hist.size()) / double(tot) : 1.0;
 }
 
+// Heavy-key lookup table of m.heavy[0, n_heavy) (see RmiParams).
+static void heavy_table_build(RmiParams &m) {
+    for (int s = 0; s < 64; s++) {
+        m.hslot_key[s] = 0;
+        m.hslot_idx[s] = -1;
+    }
+    for (int w = 0; w < 128; w++) m.hfilter[w] = 0;
+    for (uint32_t i = 0; i < m.n_heavy && i < uint32_t(kMaxHeavy); i++) {
+        const uint32_t k = m.heavy[i];
+        const uint32_t x = k * kHashMul;
+        uint32_t s = x >> 26;
+        while (m.hslot_idx[s] >= 0 && m.hslot_key[s] != k) s = (s + 1) & 63;
+        m.hslot_key[s] = k;
+        m.hslot_idx[s] = int32_t(i);
+        m.hfilter[x >> 25] |= 1u << ((x >> 20) & 31);
+    }
+}
+
 }  // namespace sjb
 
 using namespace sjb;
@@ -407,6 +425,7 @@ int sjb_rmi_fit(sjb_ctx_t *c, const sjb_tuple_t *d_train, uint64_t n_train, cons
     std::memset(&hm, 0, sizeof(hm));
     hm.n_heavy = uint32_t(runs.size());
     for (uint32_t h = 0; h < hm.n_heavy; h++) hm.heavy[h] = runs[h].second;
+    heavy_table_build(hm);
     std::vector<uint32_t> rest;
     rest.reserve(kFitSample);
     for (uint32_t k : keys) {
@@ -422,6 +441,9 @@ int sjb_rmi_fit(sjb_ctx_t *c, const sjb_tuple_t *d_train, uint64_t n_train, cons
     // header (kmin, scale, root_kind, n_heavy) and heavy keys
     SJB_HIP_TRY(hipMemcpyAsync(c->d_rmi, &hm, 16, hipMemcpyHostToDevice, s));
     SJB_HIP_TRY(hipMemcpyAsync(c->d_rmi->heavy, hm.heavy, sizeof(hm.heavy), hipMemcpyHostToDevice, s));
+    SJB_HIP_TRY(hipMemcpyAsync(c->d_rmi->hslot_key, hm.hslot_key,
+                               sizeof(hm.hslot_key) + sizeof(hm.hslot_idx) + sizeof(hm.hfilter),
+                               hipMemcpyHostToDevice, s));
     SJB_HIP_TRY(hipMemcpyAsync(c->d_fit_keys, rest.data(), size_t(ns) * 4, hipMemcpyHostToDevice, s));
     k_make_xt<<<(ns + 255) / 256, 256, 0, s>>>(c->d_fit_keys, ns, hm.kmin, hm.scale, c->d_fit_x, c->d_fit_t);
     SJB_HIP_TRY(hipGetLastError());

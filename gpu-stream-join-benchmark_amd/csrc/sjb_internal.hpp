@@ -89,6 +89,13 @@ struct RmiParams {
     float leaf_a[kRmiLeaves];       // cdf = leaf_a * x + leaf_b
     float leaf_b[kRmiLeaves];
     uint32_t heavy[kMaxHeavy];
+    // heavy-key lookup structure, built on the host (heavy_table_build) and
+    // copied into LDS by every partition workgroup: 64-slot open addressing
+    // on the top 6 bits of key * kHashMul (slot -> key, heavy index, -1 =
+    // empty) and a 4096-bit filter on the top 12 bits
+    uint32_t hslot_key[64];
+    int32_t hslot_idx[64];
+    uint32_t hfilter[128];
 };
 
 // Device parameters of range partitioning (written by k_minmax_final).
@@ -306,8 +313,8 @@ int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a
 int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint32_t *qhead,
                        uint64_t *heavy);
 size_t partition_lds_bytes_chunk(int logF, int chunk);
-size_t join_view_bytes(int alg, int ncL, int ncR);
-int join_idcap(int alg);
+size_t join_view_bytes(int alg, bool grouped, int ncL, int ncR);
+int join_idcap(int alg, bool grouped);
 
 // ---- pointer checks / output staging (api.hip) ----
 // True when the device can dereference p (device memory, or pinned /

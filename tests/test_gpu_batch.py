@@ -127,7 +127,8 @@ def test_autotune_agrees(ctx, torch_mod, sjbmod):
     dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
     base = ctx.lazy_join_batch(HJ, F_C2, dR, lo, dS, ro)
     best, tried = ctx.batch_autotune(HJ, F_C2, dR, lo, dS, ro, reps=2)
-    assert tried == 24 and best.ms > 0
+    # fanout bias {-1,0,1} x chunk {2^12,2^14} x workgroups/CU {1,2,3} x grab {1,4}
+    assert tried == 36 and best.ms > 0
     t = ctx.batch_tune()
     assert (t.fanout_bias, t.chunk_log2, t.blocks_per_cu, t.grab) == \
         (best.fanout_bias, best.chunk_log2, best.blocks_per_cu, best.grab)
@@ -138,6 +139,8 @@ def test_autotune_agrees(ctx, torch_mod, sjbmod):
 def test_tune_validation(ctx):
     with pytest.raises(sjb_error()):
         ctx.batch_tune(chunk_log2=13)
+    with pytest.raises(sjb_error()):
+        ctx.batch_tune(blocks_per_cu=4)
 
 
 def sjb_error():
