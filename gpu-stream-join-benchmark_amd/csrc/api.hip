@@ -166,7 +166,6 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
         *logF_used = 0;
         if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], c->stream));
         if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], c->stream));
-        if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
         rc = launch_nlj(c, L, nL, R, nR, flags, write, out, out_cap, wrap, d_res);
         if (rc) return rc;
         if (ev) SJB_HIP_TRY(hipEventRecord(ev[3], c->stream));
@@ -186,7 +185,9 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
         SJB_HIP_TRY(hipEventRecord(c->ov_part_done[buf], ps));
         SJB_HIP_TRY(hipStreamWaitEvent(c->stream, c->ov_part_done[buf], 0));
     }
-    if (ev) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
+    // ev[2] (join start) differs from ev[1] only when the partition pass ran
+    // on the second stream; otherwise ev[1] stands for it (one event less)
+    if (ev && buf >= 0) SJB_HIP_TRY(hipEventRecord(ev[2], c->stream));
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
@@ -549,16 +550,18 @@ int sjb_lazy_join_async(sjb_ctx_t *c, int algorithm, uint32_t flags, const sjb_t
     int logF;
     flags |= c->opts.flags;
     const hipEvent_t *ev = nullptr;
-    if ((flags & SJB_F_TIMING) && !c->tring.empty()) {
-        ev = &c->tring[4 * (c->tring_next % (c->tring.size() / 4))];
-        c->tring_next++;
-    }
     int buf = -1;
     if ((flags & SJB_F_OVERLAP) && algorithm != SJB_ALG_NLJ) {
         SJB_HIP_TRY(hipSetDevice(c->device));
         int rc = overlap_setup(c);
         if (rc) return rc;
         buf = int(c->ov_next++ & 1);
+    }
+    if ((flags & SJB_F_TIMING) && !c->tring.empty()) {
+        const uint64_t slot = c->tring_next % (c->tring.size() / 4);
+        ev = &c->tring[4 * slot];
+        c->tring_split[slot] = buf >= 0;
+        c->tring_next++;
     }
     return window_enqueue(c, algorithm, flags, d_left, n_left, d_right, n_right, 0, nullptr, 0, 0,
                           d_result, &logF, ev, buf);
@@ -570,6 +573,7 @@ int sjb_timing_ring(sjb_ctx_t *c, uint32_t n) {
     SJB_HIP_TRY(hipStreamSynchronize(c->stream));
     for (hipEvent_t e : c->tring) (void)hipEventDestroy(e);
     c->tring.clear();
+    c->tring_split.assign(n, 0);
     c->tring_next = 0;
     // Timestamp-only events: no system-scope fence (cache writeback and
     // invalidate) when they are recorded, so timing a window costs it
@@ -592,7 +596,7 @@ int sjb_timing_read(sjb_ctx_t *c, float *part_ms, float *join_ms, uint32_t n, ui
         const uint64_t w = c->tring_next - have + i;  // oldest first
         const hipEvent_t *e = &c->tring[4 * (w % slots)];
         SJB_HIP_TRY(hipEventElapsedTime(&part_ms[i], e[0], e[1]));
-        SJB_HIP_TRY(hipEventElapsedTime(&join_ms[i], e[2], e[3]));
+        SJB_HIP_TRY(hipEventElapsedTime(&join_ms[i], c->tring_split[w % slots] ? e[2] : e[1], e[3]));
     }
     *n_read = uint32_t(have);
     return SJB_OK;

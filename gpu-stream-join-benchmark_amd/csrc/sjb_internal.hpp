@@ -201,6 +201,7 @@ struct sjb_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     hipEvent_t ev[4] = {};             // per-phase timing of one call (partition start / end, join start / end)
     std::vector<hipEvent_t> tring;    // timing ring: 4 events per async window
+    std::vector<uint8_t> tring_split; // per slot: join start is its own event (ev[2])
     uint64_t tring_next = 0;
     // overlapped async windows (SJB_F_OVERLAP): the partition pass of a
     // window runs on ov_stream into buffer set (window index & 1) while the
