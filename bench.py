@@ -103,6 +103,7 @@ def learned_case(a, first, nw, torch, sjb, np):
             best, tried = ctx.batch_autotune(alg, f, dR, off, dS, off, reps=2)
             d_res = torch.zeros(nw * 32, dtype=torch.uint8, device="cuda")
             plan = ctx.batch_plan(alg, f, dR, off, dS, off, d_res)
+            pinfo = plan.info()
             ms = time_plan(ctx, plan, a.steps, a.warmup, torch)
             r = d_res.cpu().numpy().view(sjb.WRES_DTYPE).copy()
             plan.destroy()
@@ -112,6 +113,8 @@ def learned_case(a, first, nw, torch, sjb, np):
                          "partition": {k: rep.as_dict()[k] for k in ("kind", "imbalance", "fixup_frac",
                                                                     "max_build", "heavy_left")},
                          "tune": best.as_dict(),
+                         "plan_partitioning": ["hash", "radix ranges", "learned CDF"][pinfo.part_kind],
+                         "drift_fallback": bool(pinfo.drift_fallback),
                          "results": [(int(x["count"]), int(x["checksum1"])) for x in r]}
         if out["radix"]["results"] != out["learned"]["results"]:
             raise SystemExit(f"{name}: learned and radix results differ")

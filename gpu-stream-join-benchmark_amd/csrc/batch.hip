@@ -40,6 +40,7 @@ struct sjb_batch_plan {
     uint64_t *d_heavy = nullptr;      // learned + grouped: per-window heavy-key sums
     size_t cap_heavy = 0;
     bool peel = false;
+    bool drift_fallback = false;      // learned CDF judged stale on this batch
     uint32_t n_heavy = 0;
     size_t cap_win = 0, cap_chunks = 0, cap_items = 0, cap_part = 0, cap_offs = 0;
     uint64_t dev_bytes = 0;
@@ -108,6 +109,22 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     }
     const uint64_t nl_tot = loff[n] - loff[0], nr_tot = roff[n] - roff[0];
     if (nl_tot + nr_tot >= (1ull << 32)) return SJB_E_CAPACITY;
+    // The learned CDF judges itself on this batch's keys: a strided sample
+    // of the left stream; if it partitions worse than radix ranges (stale
+    // after a drift), the batch uses radix ranges (peeling stays).
+    p->drift_fallback = false;
+    if (p->kind == PART_LEARNED && nl_tot > 0) {
+        const uint32_t ns = uint32_t(std::min<uint64_t>(nl_tot, 8192));
+        const uint64_t stride = nl_tot / ns;
+        std::vector<uint32_t> keys(ns);
+        SJB_HIP_TRY(hipMemcpy2DAsync(keys.data(), sizeof(uint32_t), L + loff[0], stride * sizeof(sjb_tuple_t),
+                                     sizeof(uint32_t), ns, hipMemcpyDeviceToHost, c->stream));
+        SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (rmi_drifted(c, keys, p->peel)) {
+            p->kind = PART_RANGE;
+            p->drift_fallback = true;
+        }
+    }
 
     struct W {
         uint32_t nL, nR, ncL, ncR, logF;
@@ -402,6 +419,8 @@ int sjb_batch_plan_info(sjb_batch_plan_t *p, sjb_batch_info_t *info) {
     info->n_items = p->n_items;
     info->blocks = uint32_t(p->blocks);
     info->device_bytes = p->dev_bytes;
+    info->part_kind = uint32_t(p->kind);
+    info->drift_fallback = p->drift_fallback ? 1u : 0u;
     info->tune.fanout_bias = p->tune.fanout_bias;
     info->tune.chunk_log2 = uint32_t(p->tune.chunk_log2);
     info->tune.blocks_per_cu = uint32_t(p->tune.blocks_per_cu);

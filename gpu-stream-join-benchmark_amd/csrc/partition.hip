@@ -88,15 +88,31 @@ __device__ __forceinline__ void heavy_drain(HeavySet &hs, const HeavyQ &q, uint3
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
-// Copies the model's precomputed lookup table (one word per thread) and
-// clears the counters; the caller puts a barrier after it.
-__device__ __forceinline__ void heavy_set_load(HeavySet &h, const RmiParams *__restrict__ m) {
+// The model's precomputed lookup table, one word (or two) per thread: the
+// global loads are issued first (heavy_fetch), the chunk's own loads next,
+// and the LDS copy (heavy_store, which also clears the counters) waits
+// only for the table; the caller puts a barrier after heavy_store.
+struct HeavyFetch {
+    uint32_t key, filter;
+    int idx;
+};
+__device__ __forceinline__ HeavyFetch heavy_fetch(const RmiParams *__restrict__ m) {
+    const uint32_t t = threadIdx.x;
+    HeavyFetch f{0u, 0u, -1};
+    if (t < 64) {
+        f.key = m->hslot_key[t];
+        f.idx = m->hslot_idx[t];
+    }
+    if (t < 128) f.filter = m->hfilter[t];
+    return f;
+}
+__device__ __forceinline__ void heavy_store(HeavySet &h, const HeavyFetch &f) {
     const uint32_t t = threadIdx.x;
     if (t < 64) {
-        h.key[t] = m->hslot_key[t];
-        h.idx[t] = m->hslot_idx[t];
+        h.key[t] = f.key;
+        h.idx[t] = f.idx;
     }
-    if (t < 128) h.filter[t] = m->hfilter[t];
+    if (t < 128) h.filter[t] = f.filter;
     if (t < kMaxHeavy) {
         h.cnt[t] = 0;
         h.sum[t] = 0;
@@ -237,8 +253,8 @@ template <int KIND, int PAY, int CHUNK, bool PEEL>
 __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ in, uint64_t base,
                                                 int len, int logF, const PartFn<KIND> &part,
                                                 uint32_t *__restrict__ o, Rec *__restrict__ out,
-                                                uint32_t *lds, HeavySet *hs, bool left,
-                                                uint64_t *__restrict__ heavy_acc) {
+                                                uint32_t *lds, HeavySet *hs, const RmiParams *rmi,
+                                                bool left, uint64_t *__restrict__ heavy_acc) {
     const int F = 1 << logF;
     const int NB = F + (PEEL ? 1 : 0);                     // buckets
     uint32_t *hist = lds;                                  // NB + 1 counters
@@ -246,7 +262,8 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
     Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 2 + kPartThreads / kWave + 1) & ~1));
 
     for (int q = threadIdx.x; q <= NB; q += kPartThreads) hist[q] = 0;
-    __syncthreads();
+    HeavyFetch hf{0u, 0u, -1};
+    if (PEEL) hf = heavy_fetch(rmi);
     constexpr int kPer = CHUNK / kPartThreads;
     Rec rec[kPer];
     uint32_t pid[kPer], rank[kPer];
@@ -265,6 +282,8 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
             rec[j] = make_uint2(kv.x, PAY == PAY_VAL ? kv.y : uint32_t(base + e));
         }
     }
+    if (PEEL) heavy_store(*hs, hf);
+    __syncthreads();  // zeroed histogram (and the heavy-key table) visible
     uint32_t qn = 0;  // queued heavy hits of this wave (wave-uniform)
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
@@ -356,10 +375,6 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
     uint32_t *__restrict__ offs, Rec *__restrict__ part_out, uint64_t *__restrict__ heavy_acc) {
     extern __shared__ uint32_t lds[];
     __shared__ HeavySet hs[1];
-    if (PEEL) {
-        heavy_set_load(hs[0], rmi);
-        __syncthreads();
-    }
     PartFn<KIND> part;
     part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
     const bool left = int(blockIdx.x) < ncL;
@@ -369,7 +384,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(
     const int len = int(min<uint64_t>(kChunk, n - base));
     partition_chunk<KIND, PAY, kChunk, PEEL>(left ? L : R, base, len, logF, part,
                                              offs + uint64_t(blockIdx.x) * ((1 << logF) + 1),
-                                             part_out + (left ? 0 : nL) + base, lds, hs, left,
+                                             part_out + (left ? 0 : nL) + base, lds, hs, rmi, left,
                                              heavy_acc);
 }
 
@@ -609,10 +624,6 @@ __global__ __launch_bounds__(kPartThreads) void k_partition_batch(BatchArgs a) {
     const bool left = local < bw.ncL;
     const uint32_t chunk = left ? local : local - bw.ncL;
     const int logF = int(bw.logF);
-    if (PEEL) {
-        heavy_set_load(hs[0], a.rmi);
-        __syncthreads();
-    }
     uint32_t kmin = 0, shift = 0;
     if (KIND == PART_RANGE)
         sample_range(a.L + bw.lo, bw.nL, a.R + bw.ro, bw.nR, logF, s_mm, &kmin, &shift);
@@ -624,7 +635,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition_batch(BatchArgs a) {
     partition_chunk<KIND, PAY_VAL, CHUNK, PEEL>(
         left ? a.L + bw.lo : a.R + bw.ro, base, len, logF, part,
         a.offs + bw.offs_base + uint64_t(local) * ((1u << logF) + 1),
-        a.part + (left ? bw.lpart : bw.rpart) + base, lds, hs, left,
+        a.part + (left ? bw.lpart : bw.rpart) + base, lds, hs, a.rmi, left,
         PEEL ? a.heavy + uint64_t(bw.id) * kMaxHeavy * 4 : nullptr);
 }
 
@@ -750,6 +761,9 @@ int launch_partition_batch(hipStream_t s, int kind, bool peel, const BatchArgs &
             return big ? partition_batch_kind<PART_HASH, kChunk, false>(s, a, n_chunks, dyn)
                        : partition_batch_kind<PART_HASH, 4096, false>(s, a, n_chunks, dyn);
         case PART_RANGE:
+            if (peel)
+                return big ? partition_batch_kind<PART_RANGE, kChunk, true>(s, a, n_chunks, dyn)
+                           : partition_batch_kind<PART_RANGE, 4096, true>(s, a, n_chunks, dyn);
             return big ? partition_batch_kind<PART_RANGE, kChunk, false>(s, a, n_chunks, dyn)
                        : partition_batch_kind<PART_RANGE, 4096, false>(s, a, n_chunks, dyn);
         case PART_LEARNED:
@@ -822,6 +836,8 @@ int partition_setup(sjb_ctx *c) {
         (const void *)k_partition<PART_LEARNED, PAY_VAL, true>,
         (const void *)k_partition_batch<PART_HASH, kChunk, false>,
         (const void *)k_partition_batch<PART_RANGE, kChunk, false>,
+        (const void *)k_partition_batch<PART_RANGE, kChunk, true>,
+        (const void *)k_partition_batch<PART_RANGE, 4096, true>,
         (const void *)k_partition_batch<PART_LEARNED, kChunk, false>,
         (const void *)k_partition_batch<PART_LEARNED, kChunk, true>,
         (const void *)k_partition_batch<PART_HASH, 4096, false>,

@@ -381,6 +381,39 @@ static void heavy_table_build(RmiParams &m) {
     }
 }
 
+
+// Drift check of the learned CDF on the keys a batch will partition: the
+// partition imbalance (max / mean over Fv parts) of a strided key sample
+// under the model against MSD radix on the sample's key range; heavy keys
+// are left out when they are peeled.  Returns true when the model is more
+// than 1.25x as imbalanced as radix (the batch then uses radix ranges).
+bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel) {
+    if (c->rmi_host.size() != sizeof(RmiParams) || keys.size() < 256) return false;
+    const RmiParams &m = *reinterpret_cast<const RmiParams *>(c->rmi_host.data());
+    std::vector<uint32_t> rest;
+    rest.reserve(keys.size());
+    for (uint32_t k : keys) {
+        bool heavy = false;
+        for (uint32_t h = 0; peel && h < m.n_heavy; h++) heavy |= m.heavy[h] == k;
+        if (!heavy) rest.push_back(k);
+    }
+    if (rest.size() < 256) return false;
+    const uint32_t Fv = 256;
+    uint32_t kmin = rest[0], kmax = rest[0];
+    for (uint32_t k : rest) {
+        kmin = std::min(kmin, k);
+        kmax = std::max(kmax, k);
+    }
+    const uint32_t span = kmax - kmin;
+    const int bits = span ? 32 - __builtin_clz(span) : 0;
+    const int shift = bits > 8 ? bits - 8 : 0;
+    std::vector<uint32_t> hl(Fv, 0), hr(Fv, 0);
+    for (uint32_t k : rest) {
+        hl[host_partition(m, k, Fv)]++;
+        hr[std::min<uint32_t>((k - kmin) >> shift, Fv - 1)]++;
+    }
+    return imbalance(hl) > 1.25 * imbalance(hr);
+}
 }  // namespace sjb
 
 using namespace sjb;
@@ -524,8 +557,11 @@ int sjb_rmi_fit(sjb_ctx_t *c, const sjb_tuple_t *d_train, uint64_t n_train, cons
     rep->fallback = rep->imbalance_learned > 1.25 * rep->imbalance_radix ? 1u : 0u;
     c->rmi_valid = rep->fallback == 0;
     c->rmi_nheavy = c->rmi_valid ? m.n_heavy : 0;
+    c->rmi_host.resize(sizeof(RmiParams));
+    std::memcpy(c->rmi_host.data(), &m, sizeof(RmiParams));
     return SJB_OK;
 }
+
 
 int sjb_rmi_clear(sjb_ctx_t *c) {
     if (!c) return SJB_E_INVALID;

@@ -189,6 +189,7 @@ struct sjb_ctx {
     sjb::RmiParams *d_rmi = nullptr;
     bool rmi_valid = false;
     uint32_t rmi_nheavy = 0;          // host copy of d_rmi->n_heavy
+    std::vector<uint8_t> rmi_host;    // host copy of *d_rmi (a RmiParams) after a fit
     // learned-model fit buffers (rmi.hip), allocated on the first fit
     uint32_t *d_fit_keys = nullptr;   // sorted key sample
     float *d_fit_x = nullptr, *d_fit_t = nullptr;
@@ -315,6 +316,7 @@ int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint
                        uint64_t *heavy);
 size_t partition_lds_bytes_chunk(int logF, int chunk);
 size_t join_view_bytes(int alg, bool grouped, int ncL, int ncR);
+bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel);
 int join_idcap(int alg, bool grouped);
 
 // ---- pointer checks / output staging (api.hip) ----

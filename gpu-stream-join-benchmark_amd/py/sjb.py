@@ -65,7 +65,8 @@ class BatchTune(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     _fields_ = [("n_windows", ctypes.c_uint32), ("n_chunks", ctypes.c_uint32),
                 ("n_items", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
-                ("device_bytes", ctypes.c_uint64), ("tune", BatchTune)]
+                ("device_bytes", ctypes.c_uint64), ("tune", BatchTune),
+                ("part_kind", ctypes.c_uint32), ("drift_fallback", ctypes.c_uint32)]
 
 
 class RmiConfig(ctypes.Structure):

@@ -229,6 +229,11 @@ typedef struct sjb_batch_info {
     uint32_t blocks;        /* persistent join workgroups                    */
     uint64_t device_bytes;  /* device memory held by the plan                */
     sjb_batch_tune_t tune;
+    uint32_t part_kind;     /* partitioning used: 0 hash, 1 radix ranges,    */
+                            /* 2 learned CDF                                 */
+    uint32_t drift_fallback;/* 1: SJB_F_LEARNED asked, but the model was     */
+                            /* more imbalanced than radix ranges on a sample */
+                            /* of this batch's keys, so ranges are used      */
 } sjb_batch_info_t;
 
 int sjb_batch_plan_create(sjb_ctx_t *ctx, int algorithm, uint32_t flags,

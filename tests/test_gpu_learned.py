@@ -141,3 +141,32 @@ def test_trained_network_gains(sjbmod, torch_mod):
     assert mlp_fit.loss_final < mlp_fit.loss_init / 10
     assert mlp.imbalance < lin.imbalance < radix.imbalance
     assert mlp.fixup_frac < lin.fixup_frac < radix.fixup_frac
+
+
+def test_drift_judge_falls_back_to_ranges(fitted, torch_mod, sjbmod):
+    # The model was fitted on windows 0..3; 640 windows later the mixture
+    # means have moved by 10% of the key range and the learned CDF is
+    # stale: the batch plan judges it on a sample of its own keys and uses
+    # radix ranges instead (heavy keys stay peeled); results do not change.
+    c, _, _, _ = fitted
+    n, nw = 1 << 16, 4
+    lo = np.arange(nw + 1, dtype=np.uint64) * np.uint64(n)
+    for first, stale in ((4, False), (640, True)):
+        R = np.concatenate([sjbmod.generate_mix(n, w, seed=931 + w) for w in range(first, first + nw)])
+        S = np.concatenate([sjbmod.generate_mix(n, w, seed=951 + w) for w in range(first, first + nw)])
+        dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+        got = {}
+        for f in (F_GROUPED | F_LEARNED, F_GROUPED):
+            d_res = torch_mod.zeros(nw * 32, dtype=torch_mod.uint8, device="cuda")
+            plan = c.batch_plan(SMJ, f, dR, lo, dS, lo, d_res)
+            plan.launch()
+            c.sync()
+            info = plan.info()
+            if f & F_LEARNED:
+                assert info.drift_fallback == int(stale), (first, info.part_kind)
+                assert info.part_kind == (1 if stale else 2)
+            else:
+                assert info.drift_fallback == 0 and info.part_kind == 1
+            got[f] = d_res.cpu().numpy().view(sjbmod.WRES_DTYPE)[["count", "checksum1"]].copy()
+            plan.destroy()
+        assert np.array_equal(got[F_GROUPED | F_LEARNED], got[F_GROUPED])
