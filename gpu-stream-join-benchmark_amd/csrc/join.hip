@@ -148,12 +148,14 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     v.n[0] = nleft;
     v.n[1] = carry - nleft;
     __syncthreads();
-    for (int side = 0; side < 2; side++) {
-        for (int c = threadIdx.x; c < v.nc[side]; c += blockDim.x) {
-            const uint32_t a = v.pref[side][c];
-            const uint32_t b = min(v.pref[side][c + 1], idcap);
-            for (uint32_t j = a; j < b; j++) v.runid[side][j] = uint16_t(c);
-        }
+    // run-id fill: one thread per run of either side (slot q as above), so
+    // both sides fill at once
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        const int side = q >= ncL;
+        const int c = side ? q - ncL : q;
+        const uint32_t a = v.pref[side][c];
+        const uint32_t b = min(v.pref[side][c + 1], idcap);
+        for (uint32_t j = a; j < b; j++) v.runid[side][j] = uint16_t(c);
     }
     __syncthreads();
     return v;
@@ -916,11 +918,23 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
-    const Acc r = block_sum(acc, s_red);
-    if (threadIdx.x == 0) {
-        partials[3 * p] = r.count;
-        partials[3 * p + 1] = r.c1;
-        partials[3 * p + 2] = r.c2;
+    // block sum with a single barrier (nothing runs after it)
+    acc.count = wave_sum(acc.count);
+    acc.c1 = wave_sum(acc.c1);
+    acc.c2 = wave_sum(acc.c2);
+    constexpr int kW = kJoinThreads / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int w = threadIdx.x / kWave;
+        s_red[w] = acc.count;
+        s_red[kW + w] = acc.c1;
+        s_red[2 * kW + w] = acc.c2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kW; w++) t += s_red[threadIdx.x * kW + w];
+        partials[3 * p + threadIdx.x] = t;
     }
 }
 
