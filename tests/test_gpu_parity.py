@@ -209,6 +209,22 @@ def test_full_size_window(ctx, torch_mod, orc, sjbmod):
     assert 0.9 * 2**20 < o.count < 1.1 * 2**20
 
 
+def test_ragged_multichunk_windows(ctx, torch_mod, orc, sjbmod):
+    # more 16384-tuple chunks than partition workgroups (one per CU), both
+    # sides ending in a partial chunk and of different lengths: the
+    # persistent partition pass walks several chunks per workgroup, with
+    # the next chunk's loads in flight
+    nL, nR = 3 * (1 << 20) + 12345, (1 << 22) - 777
+    R = sjbmod.generate(nL, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=321)
+    S = sjbmod.generate(nR, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=654)
+    o, _ = orc.lazy_hj(R, S, 1 << 20)
+    for alg in (HJ, SMJ):
+        st = run_count(ctx, torch_mod, alg, R, S)
+        assert (st.result_count, st.checksum1, st.checksum2) == o.triple(), alg
+        st = run_count(ctx, torch_mod, alg, R, S, flags=F_GROUPED)
+        assert (st.result_count, st.checksum1) == (o.count, o.c1), alg
+
+
 @pytest.mark.parametrize("alg", [HJ, SMJ])
 def test_overlap_async_equals_plain(sjbmod, torch_mod, orc, alg):
     # back-to-back async windows with the partition pass of window w+1 on a
