@@ -161,6 +161,66 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     return v;
 }
 
+// load_view when ncL + ncR <= blockDim and ncL is a whole number of waves
+// (every window of up to 2^22 tuples per side with 64-aligned chunk counts,
+// e.g. 2^22 = 256 chunks): one slot per thread, one scan, the left total
+// from the wave sums, three barriers instead of six.  Ends with a barrier.
+__device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_t *__restrict__ offs, int logF,
+                                                    int p, int ncL, int ncR, uint32_t lbase, uint32_t rbase,
+                                                    uint32_t chunk, uint32_t idcap, uint32_t *ws) {
+    PartView v;
+    v.nc[0] = ncL;
+    v.nc[1] = ncR;
+    v.start[0] = dyn;
+    v.start[1] = dyn + ncL;
+    v.pref[0] = dyn + ncL + ncR;
+    v.pref[1] = v.pref[0] + ncL + 1;
+    v.runid[0] = reinterpret_cast<uint16_t *>(v.pref[1] + ncR + 1);
+    v.runid[1] = v.runid[0] + idcap;
+    v.idcap = idcap;
+    const int nq = ncL + ncR;
+    const int q = threadIdx.x;
+    const bool right = q >= ncL;
+    const int c = right ? q - ncL : q;
+    uint32_t a = 0, b = 0;
+    if (q < nq) {
+        const uint32_t *o = offs + uint64_t(q) * ((1 << logF) + 1);
+        a = o[p];
+        b = o[p + 1];
+    }
+    uint32_t wt;
+    const uint32_t ex = wave_excl_scan(b - a, &wt);
+    const int wid = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[wid] = wt;
+    __syncthreads();
+    uint32_t pre = 0, nleft = 0, tot = 0;
+    const int lw = ncL / kWave;
+    for (int w = 0; w < int(blockDim.x / kWave); w++) {
+        const uint32_t x = ws[w];
+        if (w < wid) pre += x;
+        if (w < lw) nleft += x;
+        tot += x;
+    }
+    if (q < nq) {
+        v.start[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a;
+        v.pref[right][c] = pre + ex - (right ? nleft : 0u);
+    }
+    if (threadIdx.x == 0) {
+        v.pref[0][ncL] = nleft;
+        v.pref[1][ncR] = tot - nleft;
+    }
+    v.n[0] = nleft;
+    v.n[1] = tot - nleft;
+    __syncthreads();
+    if (q < nq) {
+        const uint32_t lo = v.pref[right][c];
+        const uint32_t hi = min(v.pref[right][c + 1], idcap);
+        for (uint32_t j = lo; j < hi; j++) v.runid[right][j] = uint16_t(c);
+    }
+    __syncthreads();
+    return v;
+}
+
 __device__ __forceinline__ void write_row(sjb_result_tuple_t *__restrict__ out, uint64_t pos,
                                           uint64_t cap, int wrap, const sjb_tuple_t &l,
                                           const sjb_tuple_t &r) {
@@ -914,7 +974,10 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
     extern __shared__ uint32_t dyn[];
 
     const int p = xcd_partition(blockIdx.x, 1 << logF);
-    const PartView v = load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws);
+    const PartView v =
+        ncL + ncR <= kJoinThreads && ncL % kWave == 0
+            ? load_view_small(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws)
+            : load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws);
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
