@@ -112,6 +112,8 @@ __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
     __shared__ uint32_t s_cnt[kSlideLds + 1];
     __shared__ uint64_t s_sum[kSlideLds + 1];
     __shared__ uint64_t s_red[2 * kSlideThreads / kWave];
+    __shared__ uint16_t s_occ[kSlideLds + 1];  // occupied group slots, compacted
+    __shared__ uint32_t s_ws[kSlideThreads / kWave];
     for (int j = threadIdx.x; j <= kSlideLds; j += kSlideThreads) {
         s_key[j] = kSlideEmpty;
         s_cnt[j] = 0;
@@ -137,10 +139,32 @@ __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
         atomicAdd(reinterpret_cast<unsigned long long *>(&s_sum[sl]), (unsigned long long)h);
     }
     __syncthreads();
+    // compact the occupied slots, so each thread takes at most one distinct
+    // key when the group has at most kSlideThreads of them (its global
+    // lookup and update are one dependent chain, not two or three)
+    {
+        constexpr int kPerT = (kSlideLds + kSlideThreads) / kSlideThreads;  // slots per thread (incl. the extra)
+        uint32_t occ = 0;
+#pragma unroll
+        for (int r = 0; r < kPerT; r++) {
+            const int j = threadIdx.x * kPerT + r;
+            occ += (j <= kSlideLds && s_cnt[j] != 0) ? 1u : 0u;
+        }
+        uint32_t tot;
+        uint32_t pos = block_excl_scan_u32(occ, s_ws, &tot);
+#pragma unroll
+        for (int r = 0; r < kPerT; r++) {
+            const int j = threadIdx.x * kPerT + r;
+            if (j <= kSlideLds && s_cnt[j] != 0) s_occ[pos++] = uint16_t(j);
+        }
+        if (threadIdx.x == 0) s_ws[0] = tot;
+    }
+    __syncthreads();
+    const uint32_t n_occ = s_ws[0];
     uint64_t cnt = 0, c1 = 0;
-    for (int j = threadIdx.x; j <= kSlideLds; j += kSlideThreads) {
+    for (uint32_t t = threadIdx.x; t < n_occ; t += kSlideThreads) {
+        const int j = s_occ[t];
         const uint32_t cb = s_cnt[j];
-        if (cb == 0) continue;
         const uint32_t k = j == kSlideLds ? kSlideEmpty : s_key[j];
         const uint64_t sb = s_sum[j];
         if (PROBE) {
