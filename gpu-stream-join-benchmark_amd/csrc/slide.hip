@@ -49,45 +49,56 @@ __device__ __forceinline__ uint32_t slide_home(uint32_t k, int log_cap) {
     return uint32_t((uint64_t(k) * 0x9E3779B97F4A7C15ULL) >> (64 - log_cap));
 }
 
-__device__ __forceinline__ uint32_t slide_find_insert(SlideSlot *__restrict__ t, int log_cap, uint32_t k,
-                                                      uint32_t *__restrict__ used) {
+// A whole slot in one 16-byte load (key, count and sum arrive together).
+__device__ __forceinline__ SlideSlot slide_load(const SlideSlot *p) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    SlideSlot r;
+    r.key = v.x;
+    r.cnt = v.y;
+    r.sum = uint64_t(v.z) | (uint64_t(v.w) << 32);
+    return r;
+}
+
+// Count and sum of key k in table t (0 if absent), probing linearly from
+// its home slot s, whose content `first` is already loaded.
+__device__ __forceinline__ void slide_lookup_from(const SlideSlot *__restrict__ t, int log_cap, uint32_t k,
+                                                  uint32_t s, SlideSlot first, uint32_t *cnt, uint64_t *sum) {
     const uint32_t cap = 1u << log_cap;
-    if (k == kSlideEmpty) return cap;
-    uint32_t s = slide_home(k, log_cap);
+    SlideSlot cur = first;
     for (;;) {
-        uint32_t cur = __hip_atomic_load(&t[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur.key == k) {
+            *cnt = cur.cnt;
+            *sum = cur.sum;
+            return;
+        }
+        if (cur.key == kSlideEmpty) {
+            *cnt = 0;
+            *sum = 0;
+            return;
+        }
+        s = (s + 1) & (cap - 1);
+        cur = slide_load(&t[s]);
+    }
+}
+
+// Slot of key k in table t, inserted if absent (*fresh set when this call
+// inserted it), probing linearly from its home slot s, whose key `first`
+// is already loaded.
+__device__ __forceinline__ uint32_t slide_find_insert_from(SlideSlot *__restrict__ t, int log_cap, uint32_t k,
+                                                           uint32_t s, uint32_t first, bool *fresh) {
+    const uint32_t cap = 1u << log_cap;
+    uint32_t cur = first;
+    for (;;) {
         if (cur == kSlideEmpty) {
             cur = atomicCAS(&t[s].key, kSlideEmpty, k);
             if (cur == kSlideEmpty) {
-                atomicAdd(used, 1u);
+                *fresh = true;
                 return s;
             }
         }
         if (cur == k) return s;
         s = (s + 1) & (cap - 1);
-    }
-}
-
-__device__ __forceinline__ void slide_lookup(const SlideSlot *__restrict__ t, int log_cap, uint32_t k,
-                                             uint32_t *cnt, uint64_t *sum) {
-    const uint32_t cap = 1u << log_cap;
-    *cnt = 0;
-    *sum = 0;
-    if (k == kSlideEmpty) {
-        *cnt = t[cap].cnt;
-        *sum = t[cap].sum;
-        return;
-    }
-    uint32_t s = slide_home(k, log_cap);
-    for (;;) {
-        const uint32_t cur = t[s].key;
-        if (cur == k) {
-            *cnt = t[s].cnt;
-            *sum = t[s].sum;
-            return;
-        }
-        if (cur == kSlideEmpty) return;
-        s = (s + 1) & (cap - 1);
+        cur = __hip_atomic_load(&t[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -162,22 +173,42 @@ __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
     __syncthreads();
     const uint32_t n_occ = s_ws[0];
     uint64_t cnt = 0, c1 = 0;
+    uint32_t fresh_n = 0;
     for (uint32_t t = threadIdx.x; t < n_occ; t += kSlideThreads) {
         const int j = s_occ[t];
         const uint32_t cb = s_cnt[j];
         const uint32_t k = j == kSlideLds ? kSlideEmpty : s_key[j];
         const uint64_t sb = s_sum[j];
-        if (PROBE) {
-            uint32_t oc;
-            uint64_t os;
-            slide_lookup(other, log_cap, k, &oc, &os);
-            cnt += uint64_t(cb) * oc;
-            c1 += sb * os;
+        uint32_t g;
+        if (k == kSlideEmpty) {  // key 0xFFFFFFFF: the extra slot `cap` of both tables
+            g = 1u << log_cap;
+            if (PROBE) {
+                cnt += uint64_t(cb) * other[g].cnt;
+                c1 += sb * other[g].sum;
+            }
+        } else {
+            // the first probe of both tables (same home slot) in flight at once
+            const uint32_t s0 = slide_home(k, log_cap);
+            SlideSlot fo{};
+            if (PROBE) fo = slide_load(&other[s0]);
+            const uint32_t fk = __hip_atomic_load(&own[s0].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (PROBE) {
+                uint32_t oc;
+                uint64_t os;
+                slide_lookup_from(other, log_cap, k, s0, fo, &oc, &os);
+                cnt += uint64_t(cb) * oc;
+                c1 += sb * os;
+            }
+            bool fresh = false;
+            g = slide_find_insert_from(own, log_cap, k, s0, fk, &fresh);
+            fresh_n += fresh;
         }
-        const uint32_t g = slide_find_insert(own, log_cap, k, used);
         atomicAdd(&own[g].cnt, uint32_t(uint64_t(cb) * sign));
         atomicAdd(reinterpret_cast<unsigned long long *>(&own[g].sum), (unsigned long long)(sb * sign));
     }
+    // new keys counted once per wave instead of one global atomic each
+    const uint64_t fw = wave_sum(uint64_t(fresh_n));
+    if ((threadIdx.x & (kWave - 1)) == 0 && fw) atomicAdd(used, uint32_t(fw));
     if (PROBE) {
         cnt = wave_sum(cnt * sign);
         c1 = wave_sum(c1 * sign);
