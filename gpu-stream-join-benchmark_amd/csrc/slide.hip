@@ -111,8 +111,16 @@ __device__ __forceinline__ uint32_t slide_find_insert_from(SlideSlot *__restrict
 // key of the group instead of per tuple, so hot keys cost no atomic
 // contention.  Tuples are also copied into ring[(dst + i) % W] (if ring).
 // Sign is +1 or -1 (two's complement).
-constexpr int kSlideThreads = 1024;
-constexpr int kSlideLds = 2048;  // LDS group slots: at most 1024 keys, load <= 1/2
+// Tuples (threads) per step workgroup: larger groups combine more updates
+// of hot keys, smaller ones spread a batch over more CUs; 512 measured
+// best for batches of 2^16 Zipf(1.0) tuples (1024: -10%, 256: -6%).
+#ifndef SJB_SLIDE_THREADS
+#define SJB_SLIDE_THREADS 512
+#endif
+constexpr int kSlideThreads = SJB_SLIDE_THREADS;
+constexpr int kSlideLds = 2 * kSlideThreads;  // LDS group slots, load <= 1/2
+constexpr int kSlideLdsLog = kSlideThreads == 1024 ? 11 : kSlideThreads == 512 ? 10 : 9;
+static_assert((1 << kSlideLdsLog) == kSlideLds, "slide LDS slots");
 template <int SIDE, bool PROBE>
 __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
     const sjb_tuple_t *__restrict__ src, uint64_t base, uint64_t src_mod, uint64_t n,
@@ -138,7 +146,7 @@ __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
         const uint64_t h = SIDE == 0 ? hash_a(t.key, t.val) : hash_b(t.key, t.val);
         uint32_t sl = kSlideLds;
         if (t.key != kSlideEmpty) {
-            sl = (t.key * kHashMul) >> (32 - 11);
+            sl = (t.key * kHashMul) >> (32 - kSlideLdsLog);
             for (;;) {
                 uint32_t cur = __hip_atomic_load(&s_key[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (cur == kSlideEmpty) cur = atomicCAS(&s_key[sl], kSlideEmpty, t.key);
