@@ -20,23 +20,27 @@
 // tuple (the reference builds 32-bit words with one atomicAdd each and
 // decodes them on the host).  Non-zero words are appended as (l, w, mask)
 // triples and decoded on the device into ResultTuple rows.
+#include <algorithm>
+
 #include "sjb_device.hpp"
 #include "sjb_internal.hpp"
 
 namespace sjb {
 
 constexpr int kNljThreads = 256;
-constexpr int kNljPer = 4;
+#ifndef SJB_NLJ_PER
+#define SJB_NLJ_PER 4
+#endif
+constexpr int kNljPer = SJB_NLJ_PER;  // right keys per thread
 constexpr int kNljRTile = kNljThreads * kNljPer;  // right tuples per workgroup
 constexpr int kNljLTile = 2048;                   // left keys per LDS tile
 constexpr int kNljMaxBlocks = 65536;
 
 template <bool C2, bool WRITE>
 __global__ __launch_bounds__(kNljThreads) void k_nlj(
-    const sjb_tuple_t *__restrict__ L, uint64_t nL, const sjb_tuple_t *__restrict__ R, uint64_t nR,
-    uint64_t l_split, uint64_t *__restrict__ partials, sjb_result_tuple_t *__restrict__ out,
-    uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
-    __shared__ __align__(16) uint32_t s_lk[kNljLTile];
+    const uint32_t *__restrict__ lkeys, const sjb_tuple_t *__restrict__ L, uint64_t nL,
+    const sjb_tuple_t *__restrict__ R, uint64_t nR, uint64_t l_split, uint64_t *__restrict__ partials,
+    sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
     __shared__ uint64_t s_red[3 * kNljThreads / kWave];
     __shared__ uint32_t s_ws[kNljThreads / kWave];
     __shared__ unsigned long long s_base;
@@ -59,30 +63,35 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
 
     for (uint64_t t0 = lbeg; t0 < lend; t0 += kNljLTile) {
         const int tn = int(min<uint64_t>(kNljLTile, lend - t0));
-        for (int i = threadIdx.x; i < tn; i += kNljThreads) s_lk[i] = L[t0 + i].key;
-        __syncthreads();
-        uint32_t m = 0xFFFFFFFFu;
-        const int t4 = tn & ~3;
-        const uint4 *lk4 = reinterpret_cast<const uint4 *>(s_lk);
-        for (int e = 0; e < t4; e += 4) {
-            const uint4 q = lk4[e >> 2];
+        // Left keys are the same for every lane: they come in through scalar
+        // loads (16 per instruction) straight into the compare's scalar
+        // operand.  Two running minima (independent chains); min(min(m, x0),
+        // x1) folds into one v_min3 per two compares.
+        const uint32_t *__restrict__ tk = lkeys + t0;
+        uint32_t m = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu;
+        const int t16 = tn & ~31;
+        for (int e = 0; e < t16; e += 32) {
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const uint32_t lk = c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
-                m = min(m, min(lk ^ sk[0], lk ^ sk[1]));
-                m = min(m, min(lk ^ sk[2], lk ^ sk[3]));
+            for (int c = 0; c < 32; c++) {
+                const uint32_t lk = tk[e + c];
+#pragma unroll
+                for (int i = 0; i < kNljPer; i += 4) {
+                    m = min(min(m, lk ^ sk[i]), lk ^ sk[i + 1]);
+                    m2 = min(min(m2, lk ^ sk[i + 2]), lk ^ sk[i + 3]);
+                }
             }
         }
-        for (int e = t4; e < tn; e++) {
-            const uint32_t lk = s_lk[e];
+        for (int e = t16; e < tn; e++) {
+            const uint32_t lk = tk[e];
 #pragma unroll
             for (int i = 0; i < kNljPer; i++) m = min(m, lk ^ sk[i]);
         }
+        m = min(m, m2);
         // Exact pass only for lanes that saw a zero xor.
         uint32_t mine = 0;
         if (m == 0) {
             for (int e = 0; e < tn; e++) {
-                const uint32_t lk = s_lk[e];
+                const uint32_t lk = tk[e];
 #pragma unroll
                 for (int i = 0; i < kNljPer; i++) {
                     if (valid[i] && lk == sk[i]) {
@@ -105,7 +114,7 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
             if (mine) {
                 uint64_t pos = s_base + ex;
                 for (int e = 0; e < tn; e++) {
-                    const uint32_t lk = s_lk[e];
+                    const uint32_t lk = tk[e];
 #pragma unroll
                     for (int i = 0; i < kNljPer; i++) {
                         if (valid[i] && lk == sk[i]) {
@@ -131,8 +140,8 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
                     }
                 }
             }
+            __syncthreads();  // s_base is read before the next tile's reservation
         }
-        __syncthreads();
     }
     const Acc r = block_sum(acc, s_red);
     if (threadIdx.x == 0) {
@@ -141,6 +150,12 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
         partials[3 * b + 1] = r.c1;
         partials[3 * b + 2] = r.c2;
     }
+}
+
+// Keys of n tuples, compacted (4 B each) for the scalar loads of k_nlj.
+__global__ void k_nlj_keys(const sjb_tuple_t *__restrict__ L, uint64_t n, uint32_t *__restrict__ keys) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        keys[i] = L[i].key;
 }
 
 int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
@@ -162,19 +177,36 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
     const dim3 grid((unsigned)gx, (unsigned)gy);
     const bool c2 = (flags & SJB_F_C2) != 0;
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+    if (c->nlj_keys_cap < nL) {  // grow-only key buffer, owned by the ctx
+        if (c->d_nlj_keys) {
+            auto &v = c->dev_allocs;
+            v.erase(std::remove(v.begin(), v.end(), static_cast<void *>(c->d_nlj_keys)), v.end());
+            (void)hipFree(c->d_nlj_keys);
+            c->bytes -= c->nlj_keys_cap * sizeof(uint32_t);
+            c->d_nlj_keys = nullptr;
+            c->nlj_keys_cap = 0;
+        }
+        const uint64_t want = std::max<uint64_t>(nL, c->cap);
+        int rc = dev_alloc(c, (void **)&c->d_nlj_keys, want * sizeof(uint32_t));
+        if (rc) return rc;
+        c->nlj_keys_cap = want;
+    }
+    const uint32_t *lkeys = c->d_nlj_keys;
+    k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys);
+    SJB_HIP_TRY(hipGetLastError());
     if (write) {
         if (c2)
-            k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
                                                                    out, out_cap, wrap, cur);
         else
-            k_nlj<false, true><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<false, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
                                                                     out, out_cap, wrap, cur);
     } else {
         if (c2)
-            k_nlj<true, false><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<true, false><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
                                                                     out, out_cap, wrap, cur);
         else
-            k_nlj<false, false><<<grid, kNljThreads, 0, c->stream>>>(L, nL, R, nR, l_split,
+            k_nlj<false, false><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split,
                                                                      c->d_partials, out, out_cap, wrap, cur);
     }
     SJB_HIP_TRY(hipGetLastError());

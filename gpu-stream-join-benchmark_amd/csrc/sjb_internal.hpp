@@ -182,6 +182,8 @@ struct sjb_ctx {
     uint32_t *d_offs = nullptr;       // (F+1) local partition offsets per chunk
     uint64_t offs_cap = 0;
     uint32_t *d_scratch = nullptr;    // small per-call scratch (min/max partials)
+    uint32_t *d_nlj_keys = nullptr;   // NLJ: compacted left keys (grow-only)
+    uint64_t nlj_keys_cap = 0;
     uint64_t *d_partials = nullptr;   // 4 u64 per partition
     sjb_window_result_t *d_result = nullptr;
     sjb_window_result_t *h_result = nullptr;  // pinned
