@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 128 MiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", default="window", choices=["window", "batch", "learned", "slide"],
+    ap.add_argument("--workload", default="window", choices=["window", "batch", "learned", "slide", "nlj"],
                     help="window: 2^22-tuple tumbling windows (default); batch: 4096 lognormal "
                          "windows per launch, Zipf keys over [0, 2^28)")
     ap.add_argument("--theta", type=float, default=0.0, help="batch: Zipf theta (0 = uniform)")
@@ -234,6 +234,91 @@ def bench_slide(a):
     print(json.dumps(line))
 
 
+VALU_PEAK_OPS = 256 * 64 * 2.4e9  # MI355X: 256 CUs x 64 int32 lanes per clock x 2.4 GHz
+NLJ_OPS_PER_COMPARE = 1.5        # xor + half a min3 per compare
+
+
+def bench_nlj(a):
+    """Lazy nested-loop join: 64 tumbling windows of 2^16 tuples per side
+    (keys uniform in [0, 2^24), count + C1 + C2).  A step is one pass over
+    the 64 windows.  The compare is VALU work (n_L * n_R compares per
+    window), so the roofline is the VALU issue rate, not HBM."""
+    import numpy as np
+    import torch
+    import sjb
+
+    W, nw = 1 << 16, 64
+    N = W * nw
+    R = sjb.generate(N, sjb.DIST_UNIFORM, 1 << 24, 0.0, seed=123)
+    S = sjb.generate(N, sjb.DIST_UNIFORM, 1 << 24, 0.0, seed=456)
+    dR = torch.from_numpy(R.view(np.uint8)).to("cuda")
+    dS = torch.from_numpy(S.view(np.uint8)).to("cuda")
+    ctx = sjb.Context(device=0, max_window=W, distinct_keys=1 << 24)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    d_res = torch.zeros(nw * 32, dtype=torch.uint8, device="cuda")
+    br, bs, bo = dR.data_ptr(), dS.data_ptr(), d_res.data_ptr()
+    flags = 0 if a.no_c2 else sjb.F_C2
+
+    def step(evs=None):
+        for w in range(nw):
+            if evs is not None:
+                evs[w][0].record(stream)
+            ctx.lazy_join_async(sjb.ALG["NLJ"], flags, br + w * W * 16, W, bs + w * W * 16, W, bo + w * 32)
+            if evs is not None:
+                evs[w][1].record(stream)
+
+    for _ in range(a.warmup):
+        step()
+    ctx.sync()
+    first = d_res.cpu().numpy().view(sjb.WRES_DTYPE).copy()
+    # every window of the last step carries events (window = key compaction
+    # + compare + reduction); the other steps run without them
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(nw)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for s_ in range(a.steps):
+        step(evs if s_ == a.steps - 1 else None)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_start
+    win_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    last = d_res.cpu().numpy().view(sjb.WRES_DTYPE)
+    if a.warmup > 0 and not np.array_equal(first, last):
+        raise SystemExit("NLJ results changed between steps")
+    compares = float(W) * W
+    avg = float(win_ms.mean()) / 1e3
+    achieved = compares * NLJ_OPS_PER_COMPARE / avg
+    out = {
+        "metric": "NLJ compares/s (lazy nested-loop join, 2^16-tuple windows)",
+        "value": compares * nw * a.steps / t_wall, "unit": "compares/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t_wall / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 keys / u64 sums",
+        "data": "synthetic (splitmix64 generator, uniform keys)",
+        "config": {"workload": f"lazy NLJ tumbling windows 2^16/side, {nw} windows per step, keys U[0,2^24)",
+                   "checksums": "count+C1" + ("" if a.no_c2 else "+C2"), "parallelism": "single GPU"},
+        "input_tuples_per_s": 2 * N * a.steps / t_wall,
+        "window_ms": {"p50": float(np.percentile(win_ms, 50)), "p99": float(np.percentile(win_ms, 99)),
+                      "mean": float(win_ms.mean())},
+        "roofline": {"bound": "valu", "kernel": "NLJ window (key compaction + compare + reduction)",
+                     "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12, "unit": "T int32 ops/s",
+                     "frac": achieved / VALU_PEAK_OPS, "traffic": None,
+                     "ops_per_launch": compares * NLJ_OPS_PER_COMPARE, "avg_launch_ms": 1e3 * avg},
+        "total_matches": int(last["count"].sum()),
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        o, _ = oracle.lazy_nlj(R[:W], S[:W])
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": compares / secs, "unit": "compares/s", "cores": 1, "kind": "port",
+                               "sample": "window 0 (2^16 x 2^16), lazy CPU_NLJ restated in C, one thread",
+                               "seconds": secs, "count_matches_gpu": int(o.count) == int(last[0]["count"])}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def bench_batch(a):
     """4096 windows per launch, per-side sizes LogNormal(median 2^14, sigma 1)
     clipped to [2^8, 2^20], keys Zipf(theta) over [0, 2^28) (uniform at
@@ -329,6 +414,8 @@ def main():
         return bench_learned(a)
     if a.workload == "slide":
         return bench_slide(a)
+    if a.workload == "nlj":
+        return bench_nlj(a)
     import numpy as np
     import torch
     import sjb

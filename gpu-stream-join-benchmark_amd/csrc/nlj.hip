@@ -6,14 +6,15 @@
 // bitmap kernels getJoinResultBitmap / getJoinResultBitmapEager
 // (GPU_NLJ.cu:66-87, eager/executor/GPU/GPU_NLJ_BM.cu:55-76).
 //
-// Count kernel: each thread keeps 4 right (probe) tuples in registers; the
-// workgroup streams a range of left keys through LDS in 2048-key tiles and
-// every lane compares its 4 keys with every tile key (broadcast LDS reads,
-// 4 keys per ds_read_b128).  The compare is VALU work: x = lk ^ sk and
-// m = min3(m, x0, x1) — 1.5 ops per compare — so a tile with no match costs
-// nothing else.  Only lanes whose m reached 0 rescan the tile exactly and
-// add count, C1 and C2 per matching pair.  The NLJ is VALU-bound (n_L*n_R
-// compares), not HBM-bound.
+// Count kernel: each thread keeps 4 right (probe) keys in registers; the
+// left keys of the workgroup's range are compacted once into a 4-B key array
+// and come in through scalar loads (16 keys per instruction), shared by the
+// whole wave, straight into the compare's scalar operand.  The compare is
+// VALU work: x = lk ^ sk and m = min3(m, x0, x1) — 1.5 ops per compare — so
+// a tile with no match costs nothing else.  Only waves with a lane whose m
+// reached 0 rescan the tile (again 32 keys per load batch, a cheap min-xor
+// test per key) and add count, C1 and C2 per matching pair.  The NLJ is
+// VALU-bound (n_L*n_R compares), not HBM-bound.
 //
 // Bitmap method: one wave holds 64 consecutive right keys; for each left
 // key, __ballot(lk == rk) is exactly the 64-bit match word of that left
@@ -31,13 +32,59 @@ constexpr int kNljThreads = 256;
 #ifndef SJB_NLJ_PER
 #define SJB_NLJ_PER 4
 #endif
+#ifndef SJB_NLJ_LTILE
+#define SJB_NLJ_LTILE 256
+#endif
+#ifndef SJB_NLJ_WG_PER_CU
+#define SJB_NLJ_WG_PER_CU 8
+#endif
 constexpr int kNljPer = SJB_NLJ_PER;  // right keys per thread
 constexpr int kNljRTile = kNljThreads * kNljPer;  // right tuples per workgroup
-constexpr int kNljLTile = 2048;                   // left keys per LDS tile
+constexpr int kNljLTile = SJB_NLJ_LTILE;          // left keys per tile (exact-pass granularity)
 constexpr int kNljMaxBlocks = 65536;
 
+constexpr int kNljKeySlack = 64;  // keys readable past the end of the key array
+
+// Calls f(e, i) for every left key e < tn of the tile and right key i of
+// this lane that are equal.  Keys come in 32 at a time through scalar loads
+// (reads up to 31 keys past tn stay inside the key array's slack); each key
+// costs a min-xor over the lane's keys and one bit of a per-lane hit mask,
+// and only the set bits are revisited (a vector reload of that key).
+template <class F>
+__device__ __forceinline__ void nlj_tile_hits(const uint32_t *__restrict__ tk, int tn,
+                                              const uint32_t (&sk)[kNljPer], const bool (&valid)[kNljPer],
+                                              F &&f) {
+    for (int e = 0; e < tn; e += 32) {
+        uint32_t hm = 0;
+#pragma unroll
+        for (int c = 0; c < 32; c++) {
+            const uint32_t lk = tk[e + c];
+            uint32_t x = lk ^ sk[0];
+#pragma unroll
+            for (int i = 1; i < kNljPer; i++) x = min(x, lk ^ sk[i]);
+            hm |= uint32_t(x == 0) << c;
+        }
+        if (tn - e < 32) hm &= (1u << (tn - e)) - 1u;
+        while (hm) {
+            const int c = __builtin_ctz(hm);
+            hm &= hm - 1u;
+            const uint32_t lk = tk[e + c];
+#pragma unroll
+            for (int i = 0; i < kNljPer; i++)
+                if (valid[i] && lk == sk[i]) f(e + c, i);
+        }
+    }
+}
+
+#ifndef SJB_NLJ_WPE
+#define SJB_NLJ_WPE 0
+#endif
 template <bool C2, bool WRITE>
-__global__ __launch_bounds__(kNljThreads) void k_nlj(
+__global__ __launch_bounds__(kNljThreads)
+#if SJB_NLJ_WPE
+__attribute__((amdgpu_waves_per_eu(WRITE ? 1 : SJB_NLJ_WPE)))
+#endif
+void k_nlj(
     const uint32_t *__restrict__ lkeys, const sjb_tuple_t *__restrict__ L, uint64_t nL,
     const sjb_tuple_t *__restrict__ R, uint64_t nR, uint64_t l_split, uint64_t *__restrict__ partials,
     sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
@@ -90,21 +137,15 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
         // Exact pass only for lanes that saw a zero xor.
         uint32_t mine = 0;
         if (m == 0) {
-            for (int e = 0; e < tn; e++) {
-                const uint32_t lk = tk[e];
-#pragma unroll
-                for (int i = 0; i < kNljPer; i++) {
-                    if (valid[i] && lk == sk[i]) {
-                        mine++;
-                        if (!WRITE) {
-                            const uint64_t a = hash_a(lk, L[t0 + e].val);
-                            acc.count++;
-                            acc.c1 += a * hash_b(lk, sv[i]);
-                            if (C2) acc.c2 += pair_hash_from_a(a, sv[i]);
-                        }
-                    }
+            nlj_tile_hits(tk, tn, sk, valid, [&](int e, int i) {
+                mine++;
+                if (!WRITE) {
+                    const uint64_t a = hash_a(sk[i], L[t0 + e].val);
+                    acc.count++;
+                    acc.c1 += a * hash_b(sk[i], sv[i]);
+                    if (C2) acc.c2 += pair_hash_from_a(a, sv[i]);
                 }
-            }
+            });
         }
         if (WRITE) {
             uint32_t tot;
@@ -113,32 +154,26 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
             __syncthreads();
             if (mine) {
                 uint64_t pos = s_base + ex;
-                for (int e = 0; e < tn; e++) {
-                    const uint32_t lk = tk[e];
-#pragma unroll
-                    for (int i = 0; i < kNljPer; i++) {
-                        if (valid[i] && lk == sk[i]) {
-                            const sjb_tuple_t lt = L[t0 + e];
-                            const sjb_tuple_t st = R[rbase + threadIdx.x + uint64_t(i) * kNljThreads];
-                            const uint64_t a = hash_a(lt.key, lt.val);
-                            acc.count++;
-                            acc.c1 += a * hash_b(st.key, st.val);
-                            if (C2) acc.c2 += pair_hash_from_a(a, st.val);
-                            uint64_t q = pos++;
-                            if (wrap) q %= out_cap;
-                            if (wrap || q < out_cap) {
-                                sjb_result_tuple_t row;
-                                row.lkey = lt.key;
-                                row.rkey = st.key;
-                                row.lval = lt.val;
-                                row.rval = st.val;
-                                row.lts = lt.ts;
-                                row.rts = st.ts;
-                                out[q] = row;
-                            }
-                        }
+                nlj_tile_hits(tk, tn, sk, valid, [&](int e, int i) {
+                    const sjb_tuple_t lt = L[t0 + e];
+                    const sjb_tuple_t st = R[rbase + threadIdx.x + uint64_t(i) * kNljThreads];
+                    const uint64_t a = hash_a(lt.key, lt.val);
+                    acc.count++;
+                    acc.c1 += a * hash_b(st.key, st.val);
+                    if (C2) acc.c2 += pair_hash_from_a(a, st.val);
+                    uint64_t q = pos++;
+                    if (wrap) q %= out_cap;
+                    if (wrap || q < out_cap) {
+                        sjb_result_tuple_t row;
+                        row.lkey = lt.key;
+                        row.rkey = st.key;
+                        row.lval = lt.val;
+                        row.rval = st.val;
+                        row.lts = lt.ts;
+                        row.rts = st.ts;
+                        out[q] = row;
                     }
-                }
+                });
             }
             __syncthreads();  // s_base is read before the next tile's reservation
         }
@@ -162,9 +197,9 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
                uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
                sjb_window_result_t *d_out) {
     const uint64_t gx = (nR + kNljRTile - 1) / kNljRTile;
-    // Split the left side so the grid fills the chip (>= 4 workgroups per CU)
+    // Split the left side so the grid fills the chip (8 workgroups per CU)
     // without exceeding the partials buffer.
-    uint64_t want = uint64_t(c->n_cu) * 4;
+    uint64_t want = uint64_t(c->n_cu) * SJB_NLJ_WG_PER_CU;
     uint64_t gy = (want + gx - 1) / gx;
     uint64_t max_gy = (nL + kNljLTile - 1) / kNljLTile;
     if (gy > max_gy) gy = max_gy;
@@ -182,12 +217,12 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
             auto &v = c->dev_allocs;
             v.erase(std::remove(v.begin(), v.end(), static_cast<void *>(c->d_nlj_keys)), v.end());
             (void)hipFree(c->d_nlj_keys);
-            c->bytes -= c->nlj_keys_cap * sizeof(uint32_t);
+            c->bytes -= (c->nlj_keys_cap + kNljKeySlack) * sizeof(uint32_t);
             c->d_nlj_keys = nullptr;
             c->nlj_keys_cap = 0;
         }
         const uint64_t want = std::max<uint64_t>(nL, c->cap);
-        int rc = dev_alloc(c, (void **)&c->d_nlj_keys, want * sizeof(uint32_t));
+        int rc = dev_alloc(c, (void **)&c->d_nlj_keys, (want + kNljKeySlack) * sizeof(uint32_t));
         if (rc) return rc;
         c->nlj_keys_cap = want;
     }
