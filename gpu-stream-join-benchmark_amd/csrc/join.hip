@@ -48,6 +48,9 @@ constexpr int kHjProbe = SJB_HJ_PROBE;                  // probe records staged 
 #ifndef SJB_BATCH_HJ_WAVES
 #define SJB_BATCH_HJ_WAVES 6
 #endif
+#ifndef SJB_BATCH_SMJ_WAVES
+#define SJB_BATCH_SMJ_WAVES 6
+#endif
 constexpr int kHjPerP = kHjProbe / kJoinThreads;        // staged probe records per thread
 
 // Record index of element j (< pref[nc]) by binary search over the run
@@ -740,37 +743,92 @@ __device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &
 // ---------------------------------------------------------------------------
 // SMJ
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void bitonic_sort(uint64_t *s, int n) {
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const uint64_t a = s[i], b = s[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        s[i] = b;
-                        s[ixj] = a;
-                    }
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), m));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), m));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Bitonic sort of s[0, N), N = kJoinThreads * E, ascending.  Thread t holds
+// elements E*t .. E*t+E-1 in registers for the whole network: strides below
+// E are compare-exchanges inside the thread, strides up to 64*E exchange
+// with lane t ^ (j/E) of the same wave (cross-lane shuffles), and only the
+// larger strides (6 of the log2(N)(log2(N)+1)/2 stages) go through LDS, one
+// barrier each, alternating between tmp and s (lane-contiguous layout, no
+// bank conflicts).  Expects s written and a barrier passed; ends with s
+// sorted and a barrier.
+static_assert(kSmjCap == 4 * kJoinThreads, "SMJ sort tiles hold up to 4 elements per thread");
+
+template <int E>
+__device__ __forceinline__ void bitonic_sort_regs(uint64_t *s, uint64_t *tmp) {
+    constexpr int N = kJoinThreads * E;
+    constexpr int LOGN = E == 1 ? 9 : E == 2 ? 10 : 11;
+    static_assert(N == (1 << LOGN), "tile of kJoinThreads * E elements");
+    const int t = threadIdx.x;
+    uint64_t x[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) x[r] = s[E * t + r];
+    int flip = 0;  // LDS stages alternate buffers: one barrier per stage
+#pragma unroll
+    for (int lk = 1; lk <= LOGN; lk++) {
+        const int k = 1 << lk;
+#pragma unroll
+        for (int lj = lk - 1; lj >= 0; lj--) {
+            const int j = 1 << lj;
+            if (j < E) {
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    if (r & j) continue;
+                    const bool up = ((E * t + r) & k) == 0;
+                    const uint64_t a = x[r], b = x[r | j];
+                    const bool sw = (a > b) == up;
+                    x[r] = sw ? b : a;
+                    x[r | j] = sw ? a : b;
+                }
+            } else {
+                const int tj = j / E;  // partner thread distance
+                const bool keep_min = ((t & tj) == 0) == (((E * t) & k) == 0);
+                uint64_t y[E];
+                if (tj < kWave) {
+#pragma unroll
+                    for (int r = 0; r < E; r++) y[r] = shfl_xor_u64(x[r], tj);
+                } else {
+                    uint64_t *buf = flip ? s : tmp;
+                    flip ^= 1;
+#pragma unroll
+                    for (int r = 0; r < E; r++) buf[r * kJoinThreads + t] = x[r];
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < E; r++) y[r] = buf[r * kJoinThreads + (t ^ tj)];
+                }
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    const bool lt = x[r] < y[r];
+                    x[r] = (lt == keep_min) ? x[r] : y[r];
                 }
             }
-            __syncthreads();
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) s[E * t + r] = x[r];
+    __syncthreads();
+}
+
+// Elements of the sort tile holding n records: 512, 1024 or 2048.
+__device__ __forceinline__ int sort_tile_size(int n) {
+    return n <= kJoinThreads ? kJoinThreads : n <= 2 * kJoinThreads ? 2 * kJoinThreads : 4 * kJoinThreads;
 }
 
 // Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload).
-// SORT: pads to a power of two with ~0 and sorts; padding entries sort last
-// and equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit for bit, so the first
-// n sorted entries are exactly the tile.  Ends with a barrier.
+// SORT: pads to the sort tile size with ~0 and sorts (tmp: scratch of the
+// same size); padding entries sort last and equal a real (0xFFFFFFFF,
+// 0xFFFFFFFF) record bit for bit, so the first n sorted entries are exactly
+// the tile.  Ends with a barrier.
 template <bool SORT>
-__device__ __forceinline__ void gather_tile(uint64_t *s, const Rec *__restrict__ part,
+__device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, const Rec *__restrict__ part,
                                             const PartView &v, int side, uint32_t t0, int n) {
-    int np2 = n;
-    if (SORT) {
-        np2 = 2;
-        while (np2 < n) np2 <<= 1;
-    }
+    const int np2 = SORT ? sort_tile_size(n) : n;
     for (int i0 = 0; i0 < np2; i0 += kJoinThreads * kBatch) {
         Rec r[kBatch];
 #pragma unroll
@@ -785,7 +843,14 @@ __device__ __forceinline__ void gather_tile(uint64_t *s, const Rec *__restrict__
         }
     }
     __syncthreads();
-    if (SORT) bitonic_sort(s, np2);
+    if (SORT) {
+        if (np2 == kJoinThreads)
+            bitonic_sort_regs<1>(s, tmp);
+        else if (np2 == 2 * kJoinThreads)
+            bitonic_sort_regs<2>(s, tmp);
+        else
+            bitonic_sort_regs<4>(s, tmp);
+    }
 }
 
 __device__ __forceinline__ int lower_bound_key(const uint64_t *s, int n, uint32_t k) {
@@ -832,7 +897,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_tile<true>(s_b, part, v, 0, t0, tn);
+        gather_tile<true>(s_b, s_p, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -869,7 +934,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
         }
         for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
             const int un = int(min<uint32_t>(kSmjCap, np - u0));
-            gather_tile<false>(s_p, part, v, 1, u0, un);
+            gather_tile<false>(s_p, nullptr, part, v, 1, u0, un);
             if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
@@ -1018,7 +1083,9 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
 // capped to match (6 waves per SIMD -> at most 80 VGPRs).
 template <int ALG, bool C2, bool GROUPED>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kJoinThreads),
-                          amdgpu_waves_per_eu(ALG == SJB_ALG_HJ ? SJB_BATCH_HJ_WAVES : 4))) void
+                          amdgpu_waves_per_eu(ALG == SJB_ALG_HJ ? SJB_BATCH_HJ_WAVES
+                                              : GROUPED        ? 4
+                                                               : SJB_BATCH_SMJ_WAVES))) void
 k_join_batch(BatchArgs a) {
     __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, false>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
