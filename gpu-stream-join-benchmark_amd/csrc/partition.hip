@@ -201,13 +201,33 @@ __global__ __launch_bounds__(kPartThreads) void k_minmax_partial(
     }
 }
 
-__global__ void k_minmax_final(const uint32_t *__restrict__ partial, int nblocks, int logF,
-                               PartParams *__restrict__ pp) {
-    if (threadIdx.x != 0) return;
+// One workgroup of kMinmaxFinal threads folds the per-block partials (all
+// loads in flight at once, then a wave and a block reduction).
+constexpr int kMinmaxFinal = 1024;
+__global__ __launch_bounds__(kMinmaxFinal) void k_minmax_final(const uint32_t *__restrict__ partial,
+                                                               int nblocks, int logF,
+                                                               PartParams *__restrict__ pp) {
+    __shared__ uint32_t smin[kMinmaxFinal / kWave], smax[kMinmaxFinal / kWave];
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    for (int b = 0; b < nblocks; b++) {
+    for (int b = threadIdx.x; b < nblocks; b += kMinmaxFinal) {
         mn = min(mn, partial[2 * b]);
         mx = max(mx, partial[2 * b + 1]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, kWave));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, kWave));
+    }
+    const int wid = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        smin[wid] = mn;
+        smax[wid] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kMinmaxFinal / kWave; w++) {
+        mn = min(mn, smin[w]);
+        mx = max(mx, smax[w]);
     }
     if (mn > mx) mn = mx = 0;
     uint32_t span = mx - mn;
@@ -224,7 +244,7 @@ int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, 
     // The scan-block buffer doubles as scratch for the partial min/max.
     k_minmax_partial<<<blocks, kPartThreads, 0, s>>>(L, nL, R, nR, c->d_scratch);
     SJB_HIP_TRY(hipGetLastError());
-    k_minmax_final<<<1, 64, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
+    k_minmax_final<<<1, kMinmaxFinal, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
