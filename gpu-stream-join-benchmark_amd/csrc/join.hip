@@ -743,32 +743,33 @@ __device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &
 // ---------------------------------------------------------------------------
 // SMJ
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+__device__ __forceinline__ uint64_t shfl_xor_v(uint64_t v, int m) {
     const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), m));
     const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), m));
     return (uint64_t(hi) << 32) | lo;
 }
+__device__ __forceinline__ uint32_t shfl_xor_v(uint32_t v, int m) { return uint32_t(__shfl_xor(int(v), m)); }
 
-// Bitonic sort of s[0, N), N = kJoinThreads * E, ascending.  Thread t holds
-// elements E*t .. E*t+E-1 in registers for the whole network: strides below
-// E are compare-exchanges inside the thread, strides up to 64*E exchange
-// with lane t ^ (j/E) of the same wave (cross-lane shuffles), and only the
-// larger strides (6 of the log2(N)(log2(N)+1)/2 stages) go through LDS, one
-// barrier each, alternating between tmp and s (lane-contiguous layout, no
-// bank conflicts).  Expects s written and a barrier passed; ends with s
-// sorted and a barrier.
 static_assert(kSmjCap == 4 * kJoinThreads, "SMJ sort tiles hold up to 4 elements per thread");
 
-template <int E>
-__device__ __forceinline__ void bitonic_sort_regs(uint64_t *s, uint64_t *tmp) {
+// Bitonic sorting network over N = kJoinThreads * E values held in
+// registers, ascending: thread t holds elements E*t .. E*t+E-1 in x.
+// Strides below E are compare-exchanges inside the thread, strides up to
+// 64*E exchange with lane t ^ (j/E) of the same wave (cross-lane
+// shuffles), and only the larger strides (6 of the log2(N)(log2(N)+1)/2
+// stages) go through LDS, one barrier each, alternating between buf0 and
+// buf1 (lane-contiguous layout, no bank conflicts; each holds N values).
+// Ends with a barrier after the last LDS stage's reads.
+template <int E, class T>
+__device__ __forceinline__ void bitonic_network(T (&x)[E], T *buf0, T *buf1) {
     constexpr int N = kJoinThreads * E;
     constexpr int LOGN = E == 1 ? 9 : E == 2 ? 10 : 11;
     static_assert(N == (1 << LOGN), "tile of kJoinThreads * E elements");
-    const int t = threadIdx.x;
-    uint64_t x[E];
-#pragma unroll
-    for (int r = 0; r < E; r++) x[r] = s[E * t + r];
-    int flip = 0;  // LDS stages alternate buffers: one barrier per stage
+    // opaque copy of the lane index: the network's shuffle and LDS addresses
+    // are computed here, not shared with (and held live across) other code
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    int flip = 0;
 #pragma unroll
     for (int lk = 1; lk <= LOGN; lk++) {
         const int k = 1 << lk;
@@ -780,7 +781,7 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t *s, uint64_t *tmp) {
                 for (int r = 0; r < E; r++) {
                     if (r & j) continue;
                     const bool up = ((E * t + r) & k) == 0;
-                    const uint64_t a = x[r], b = x[r | j];
+                    const T a = x[r], b = x[r | j];
                     const bool sw = (a > b) == up;
                     x[r] = sw ? b : a;
                     x[r | j] = sw ? a : b;
@@ -788,12 +789,12 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t *s, uint64_t *tmp) {
             } else {
                 const int tj = j / E;  // partner thread distance
                 const bool keep_min = ((t & tj) == 0) == (((E * t) & k) == 0);
-                uint64_t y[E];
+                T y[E];
                 if (tj < kWave) {
 #pragma unroll
-                    for (int r = 0; r < E; r++) y[r] = shfl_xor_u64(x[r], tj);
+                    for (int r = 0; r < E; r++) y[r] = shfl_xor_v(x[r], tj);
                 } else {
-                    uint64_t *buf = flip ? s : tmp;
+                    T *buf = flip ? buf1 : buf0;
                     flip ^= 1;
 #pragma unroll
                     for (int r = 0; r < E; r++) buf[r * kJoinThreads + t] = x[r];
@@ -810,8 +811,44 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t *s, uint64_t *tmp) {
         }
     }
     __syncthreads();
+}
+
+// Sorts the 64-bit (key << 32 | payload) tile s[0, N) in place.  When the
+// tile's keys span less than 2^20 (kmin: smallest key) the network runs on
+// 32-bit values ((key - kmin) << 11 | position), half the shuffles and
+// compares of the 64-bit values, and the records are permuted once at the
+// end; ~0 marks padding (a real value is below 2^31).  Otherwise the
+// 64-bit values themselves are sorted.  tmp: scratch of N 64-bit values.
+// Expects s written and a barrier passed; ends with s sorted and a barrier.
+template <int E>
+__device__ __forceinline__ void sort_tile(uint64_t *s, uint64_t *tmp, int n, bool narrow, uint32_t kmin) {
+    constexpr int N = kJoinThreads * E;
+    const int t = threadIdx.x;
+    if (narrow) {
+        uint32_t x[E];
 #pragma unroll
-    for (int r = 0; r < E; r++) s[E * t + r] = x[r];
+        for (int r = 0; r < E; r++) {
+            const int i = E * t + r;
+            x[r] = i < n ? ((uint32_t(s[i] >> 32) - kmin) << 11) | uint32_t(i) : ~0u;
+        }
+        uint32_t *b32 = reinterpret_cast<uint32_t *>(tmp);
+        bitonic_network<E>(x, b32, b32 + N);
+        uint64_t v[E];
+#pragma unroll
+        for (int r = 0; r < E; r++) v[r] = x[r] == ~0u ? ~0ull : s[x[r] & 2047u];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; r++) s[E * t + r] = v[r];
+    } else {
+        uint64_t x[E];
+#pragma unroll
+        for (int r = 0; r < E; r++) x[r] = s[E * t + r];
+        // the last LDS stage reads buf1 = s only after every thread's
+        // reads above (its barrier follows them)
+        bitonic_network<E>(x, tmp, s);
+#pragma unroll
+        for (int r = 0; r < E; r++) s[E * t + r] = x[r];
+    }
     __syncthreads();
 }
 
@@ -822,13 +859,16 @@ __device__ __forceinline__ int sort_tile_size(int n) {
 
 // Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload).
 // SORT: pads to the sort tile size with ~0 and sorts (tmp: scratch of the
-// same size); padding entries sort last and equal a real (0xFFFFFFFF,
+// same size; s_mm: 2 * kJoinThreads / kWave words of LDS for the tile's key
+// bounds); padding entries sort last and equal a real (0xFFFFFFFF,
 // 0xFFFFFFFF) record bit for bit, so the first n sorted entries are exactly
 // the tile.  Ends with a barrier.
 template <bool SORT>
-__device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, const Rec *__restrict__ part,
-                                            const PartView &v, int side, uint32_t t0, int n) {
+__device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t *s_mm,
+                                            const Rec *__restrict__ part, const PartView &v, int side,
+                                            uint32_t t0, int n) {
     const int np2 = SORT ? sort_tile_size(n) : n;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
     for (int i0 = 0; i0 < np2; i0 += kJoinThreads * kBatch) {
         Rec r[kBatch];
 #pragma unroll
@@ -840,16 +880,44 @@ __device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, const Re
         for (int k = 0; k < kBatch; k++) {
             const int i = i0 + k * kJoinThreads + threadIdx.x;
             if (i < np2) s[i] = i < n ? ((uint64_t(r[k].x) << 32) | r[k].y) : ~0ull;
+            if (SORT && i < n) {
+                mn = min(mn, r[k].x);
+                mx = max(mx, r[k].x);
+            }
+        }
+    }
+    if (SORT) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = min(mn, uint32_t(__shfl_xor(int(mn), off)));
+            mx = max(mx, uint32_t(__shfl_xor(int(mx), off)));
+        }
+        constexpr int kW = kJoinThreads / kWave;
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            s_mm[threadIdx.x / kWave] = mn;
+            s_mm[kW + threadIdx.x / kWave] = mx;
         }
     }
     __syncthreads();
     if (SORT) {
-        if (np2 == kJoinThreads)
-            bitonic_sort_regs<1>(s, tmp);
-        else if (np2 == 2 * kJoinThreads)
-            bitonic_sort_regs<2>(s, tmp);
+        constexpr int kW = kJoinThreads / kWave;
+#pragma unroll
+        for (int w = 0; w < kW; w++) {
+            mn = min(mn, s_mm[w]);
+            mx = max(mx, s_mm[kW + w]);
+        }
+        // block-uniform values (the compiler cannot see that on its own):
+        // uniform branches keep the sort variants' registers apart
+        mn = __builtin_amdgcn_readfirstlane(mn);
+        const bool narrow = __builtin_amdgcn_readfirstlane(mx - mn) < (1u << 20);  // n >= 1: mn <= mx
+        const int nu = __builtin_amdgcn_readfirstlane(n);
+        const int np2u = __builtin_amdgcn_readfirstlane(np2);
+        if (np2u == kJoinThreads)
+            sort_tile<1>(s, tmp, nu, narrow, mn);
+        else if (np2u == 2 * kJoinThreads)
+            sort_tile<2>(s, tmp, nu, narrow, mn);
         else
-            bitonic_sort_regs<4>(s, tmp);
+            sort_tile<4>(s, tmp, nu, narrow, mn);
     }
 }
 
@@ -897,7 +965,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_tile<true>(s_b, s_p, part, v, 0, t0, tn);
+        gather_tile<true>(s_b, s_p, reinterpret_cast<uint32_t *>(s_pa), part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -934,7 +1002,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
         }
         for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
             const int un = int(min<uint32_t>(kSmjCap, np - u0));
-            gather_tile<false>(s_p, nullptr, part, v, 1, u0, un);
+            gather_tile<false>(s_p, nullptr, nullptr, part, v, 1, u0, un);
             if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);

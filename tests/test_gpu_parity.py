@@ -95,6 +95,34 @@ def test_materialised_pairs(ctx, torch_mod, orc, alg, method):
     assert np.array_equal(sort_rows(rows), ref_rows)
 
 
+@pytest.mark.parametrize("spread", [1 << 12, (1 << 20) - 1, 1 << 21, 1 << 32])
+@pytest.mark.parametrize("size", [(1000, 900), (3000, 2500), (5000, 4096)])
+def test_smj_sort_key_spans(ctx, torch_mod, orc, spread, size):
+    """SMJ build tiles whose keys span less / more than 2^20 (the sort runs
+    on packed 32-bit values below that span, on the 64-bit records above
+    it); keys from a pool that includes 0 and spread - 1 (0xFFFFFFFF for the
+    full range), so every tile has matches."""
+    nL, nR = size
+    print(f"case smj spans spread={spread} size={size}", flush=True)
+    rng = np.random.default_rng(nL + spread % 977)
+    pool = rng.integers(0, spread, 300, dtype=np.uint64)
+    pool[0], pool[1] = 0, spread - 1
+    pool = pool.astype(np.uint32)
+    R = make(nL, 1, 31 + nL, keys=pool[rng.integers(0, len(pool), nL)])
+    S = make(nR, 1, 37 + nR, keys=pool[rng.integers(0, len(pool), nR)])
+    o, _ = orc.lazy_hj(R, S, 1 << 12)
+    st = run_count(ctx, torch_mod, SMJ, R, S)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    if size == (3000, 2500):
+        ref_rows = sort_rows(np_pair_rows(R, S))
+        cap = len(ref_rows) + 10
+        out = torch_mod.zeros(cap * 32, dtype=torch_mod.uint8, device="cuda")
+        st = ctx.lazy_join(SMJ, AT, 0, to_dev(torch_mod, R), nL, to_dev(torch_mod, S), nR, out, cap)
+        assert st.pairs_written == len(ref_rows)
+        rows = out.cpu().numpy().view(ref_rows.dtype)[: st.pairs_written]
+        assert np.array_equal(sort_rows(rows), ref_rows)
+
+
 @pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
 def test_capacity_errors(ctx, torch_mod, alg):
     R, S = make(2000, 50, 81), make(2000, 50, 82)
