@@ -61,13 +61,17 @@ struct HeavySet {
     HeavyQ q[kPartThreads / kWave];
 };
 
-constexpr int kHeavyReg = 4;  // most frequent heavy keys summed in registers
+#ifndef SJB_HEAVY_REG
+#define SJB_HEAVY_REG 1
+#endif
+constexpr int kHeavyReg = SJB_HEAVY_REG;  // most frequent heavy keys summed in registers
+constexpr int kHeavyRegArr = kHeavyReg > 0 ? kHeavyReg : 1;
 
 // Hashes and sums the n (<= 64) queued heavy hits of the calling wave, one
 // per lane: the kHeavyReg most frequent keys into the lane's registers, the
 // others with LDS atomics.  The queue is free again afterwards.
 __device__ __forceinline__ void heavy_drain(HeavySet &hs, const HeavyQ &q, uint32_t n, bool left,
-                                            uint32_t (&hcnt)[kHeavyReg], uint64_t (&hsum)[kHeavyReg]) {
+                                            uint32_t (&hcnt)[kHeavyRegArr], uint64_t (&hsum)[kHeavyRegArr]) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     if (lane < n) {
@@ -267,8 +271,11 @@ int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, 
 // B (right side) are added to heavy_acc[4*h + (left ? 0 : 2) + {0,1}].
 // Hits are queued per wave and hashed 64 at a time (heavy_drain); heavy
 // keys are ordered by sample frequency and the kHeavyReg most frequent are
-// summed in per-thread registers, reduced once per wave (under skew they
-// are most of the hits, and LDS atomics on one address serialise).
+// summed in per-thread registers, reduced once per wave (under skew the
+// top key is a large share of the hits, and LDS atomics on one address
+// serialise); the others go to LDS atomics.  Measured on Zipf 1.2: one
+// register key is best — each further one costs a select per hit and two
+// wave reductions per chunk, more than the atomics it saves.
 template <int KIND, int PAY, int CHUNK, bool PEEL>
 __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ in, uint64_t base,
                                                 int len, int logF, const PartFn<KIND> &part,
@@ -287,8 +294,8 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
     constexpr int kPer = CHUNK / kPartThreads;
     Rec rec[kPer];
     uint32_t pid[kPer], rank[kPer];
-    uint32_t hcnt[kHeavyReg];
-    uint64_t hsum[kHeavyReg];
+    uint32_t hcnt[kHeavyRegArr];
+    uint64_t hsum[kHeavyRegArr];
 #pragma unroll
     for (int i = 0; i < kHeavyReg; i++) {
         hcnt[i] = 0;
