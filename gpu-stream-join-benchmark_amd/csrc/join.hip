@@ -945,10 +945,12 @@ __device__ __forceinline__ int upper_bound_key(const uint64_t *s, int lo, int n,
     return lo;
 }
 
+template <bool GROUPED>
 struct SmjLds {
-    uint64_t b[kSmjCap];      // sorted build tile
-    uint64_t p[kSmjCap];      // probe tile (unsorted)
-    uint64_t pa[kSmjCap + 1]; // grouped: prefix sums of A over the sorted build tile
+    uint64_t b[kSmjCap];                          // sorted build tile
+    uint64_t p[kSmjCap];                          // probe tile (unsorted)
+    uint32_t mm[2 * kJoinThreads / kWave];        // per-wave key bounds of a build tile
+    uint64_t pa[GROUPED ? kSmjCap + 1 : 1];       // grouped: prefix sums of A over the sorted build tile
 };
 
 // Joins one partition (SMJ): each build tile is bitonic-sorted in LDS and
@@ -958,14 +960,14 @@ struct SmjLds {
 // a grouped partition whose build side exceeds one tile (heavy keys) goes
 // to the aggregated build instead, which is O(n) at any skew.
 template <bool C2, bool WRITE, bool GROUPED>
-__device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &v, SmjLds &m,
+__device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &v, SmjLds<GROUPED> &m,
                                               uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
     const Rec *__restrict__ part = io.part;
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_tile<true>(s_b, s_p, reinterpret_cast<uint32_t *>(s_pa), part, v, 0, t0, tn);
+        gather_tile<true>(s_b, s_p, m.mm, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -1061,8 +1063,8 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
 template <int ALG, bool GROUPED, bool WRITE>
 struct JoinLdsBytes {
     static constexpr size_t hj = GROUPED ? sizeof(AggLds) : sizeof(HjLds<WRITE>);
-    static constexpr size_t smj = GROUPED ? (sizeof(SmjLds) > sizeof(AggLds) ? sizeof(SmjLds) : sizeof(AggLds))
-                                          : sizeof(SmjLds);
+    static constexpr size_t smj = GROUPED ? (sizeof(SmjLds<true>) > sizeof(AggLds) ? sizeof(SmjLds<true>) : sizeof(AggLds))
+                                          : sizeof(SmjLds<false>);
     static constexpr size_t value = ALG == SJB_ALG_SMJ ? smj : hj;
 };
 
@@ -1079,7 +1081,7 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            smj_join_part<C2, WRITE, GROUPED>(io, v, *reinterpret_cast<SmjLds *>(mem), s_ws, s_base, acc);
+            smj_join_part<C2, WRITE, GROUPED>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base, acc);
     }
 }
 
