@@ -170,15 +170,26 @@ struct PartFn {
 // ---------------------------------------------------------------------------
 // min / max key over both sides (range partitioning parameters)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPartThreads) void k_minmax_partial(
-    const sjb_tuple_t *__restrict__ L, uint64_t nL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, uint32_t *__restrict__ partial) {
-    __shared__ uint32_t smin[kPartThreads / kWave], smax[kPartThreads / kWave];
+// Key bounds for range partitioning from an evenly spaced sample of at most
+// kMinmaxSample keys of the window (every key when the window is that
+// small).  Keys outside the sampled bounds are clamped to the first / last
+// partition by the range function, which stays a function of the key, so
+// equal keys still meet; the sample only sets the balance.  The sample is
+// spread over many workgroups (one key per thread: the loads of one
+// workgroup alone would queue behind each other), each writing its partial
+// bounds; k_minmax_final folds them.
+constexpr int kMinmaxThreads = 1024;
+constexpr uint32_t kMinmaxSample = 1u << 16;
+__global__ __launch_bounds__(kMinmaxThreads) void k_minmax_partial(const sjb_tuple_t *__restrict__ L, uint64_t nL,
+                                                                   const sjb_tuple_t *__restrict__ R, uint32_t ns,
+                                                                   uint64_t step, uint32_t frac,
+                                                                   uint32_t *__restrict__ partial) {
+    __shared__ uint32_t smin[kMinmaxThreads / kWave], smax[kMinmaxThreads / kWave];
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    const uint64_t n = nL + nR;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += uint64_t(gridDim.x) * blockDim.x) {
-        uint32_t k = i < nL ? L[i].key : R[i - nL].key;
+    for (uint32_t q = blockIdx.x * kMinmaxThreads + threadIdx.x; q < ns; q += gridDim.x * kMinmaxThreads) {
+        // sample q sits at floor(q * n / ns) = q * step + floor(q * frac / 2^32)
+        const uint64_t i = uint64_t(q) * step + ((uint64_t(q) * frac) >> 32);
+        const uint32_t k = i < nL ? L[i].key : R[i - nL].key;
         mn = min(mn, k);
         mx = max(mx, k);
     }
@@ -193,27 +204,22 @@ __global__ __launch_bounds__(kPartThreads) void k_minmax_partial(
         smax[wid] = mx;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kPartThreads / kWave; w++) {
-            mn = min(mn, smin[w]);
-            mx = max(mx, smax[w]);
-        }
-        mn = min(mn, smin[0]);
-        mx = max(mx, smax[0]);
-        partial[2 * blockIdx.x] = mn;
-        partial[2 * blockIdx.x + 1] = mx;
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kMinmaxThreads / kWave; w++) {
+        mn = min(mn, smin[w]);
+        mx = max(mx, smax[w]);
     }
+    partial[2 * blockIdx.x] = mn;
+    partial[2 * blockIdx.x + 1] = mx;
 }
 
-// One workgroup of kMinmaxFinal threads folds the per-block partials (all
-// loads in flight at once, then a wave and a block reduction).
-constexpr int kMinmaxFinal = 1024;
-__global__ __launch_bounds__(kMinmaxFinal) void k_minmax_final(const uint32_t *__restrict__ partial,
-                                                               int nblocks, int logF,
-                                                               PartParams *__restrict__ pp) {
-    __shared__ uint32_t smin[kMinmaxFinal / kWave], smax[kMinmaxFinal / kWave];
+// One workgroup folds the per-block partials into kmin / shift.
+__global__ __launch_bounds__(kMinmaxThreads) void k_minmax_final(const uint32_t *__restrict__ partial,
+                                                                 int nblocks, int logF,
+                                                                 PartParams *__restrict__ pp) {
+    __shared__ uint32_t smin[kMinmaxThreads / kWave], smax[kMinmaxThreads / kWave];
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kMinmaxFinal) {
+    for (int b = threadIdx.x; b < nblocks; b += kMinmaxThreads) {
         mn = min(mn, partial[2 * b]);
         mx = max(mx, partial[2 * b + 1]);
     }
@@ -229,7 +235,7 @@ __global__ __launch_bounds__(kMinmaxFinal) void k_minmax_final(const uint32_t *_
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < kMinmaxFinal / kWave; w++) {
+    for (int w = 1; w < kMinmaxThreads / kWave; w++) {
         mn = min(mn, smin[w]);
         mx = max(mx, smax[w]);
     }
@@ -242,13 +248,15 @@ __global__ __launch_bounds__(kMinmaxFinal) void k_minmax_final(const uint32_t *_
 
 int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
                   uint64_t nR, int logF) {
-    uint64_t n = nL + nR;
-    int blocks = int(std::min<uint64_t>((n + kChunk - 1) / kChunk, 1024));
-    if (blocks < 1) blocks = 1;
-    // The scan-block buffer doubles as scratch for the partial min/max.
-    k_minmax_partial<<<blocks, kPartThreads, 0, s>>>(L, nL, R, nR, c->d_scratch);
+    const uint64_t n = nL + nR;
+    const uint32_t ns = uint32_t(std::min<uint64_t>(n, kMinmaxSample));
+    const int blocks = std::max(1, int((ns + kMinmaxThreads - 1) / kMinmaxThreads));
+    const uint64_t step = ns ? n / ns : 0;
+    const uint32_t frac = ns ? uint32_t(((n % ns) << 32) / ns) : 0u;
+    // The scan-block buffer doubles as scratch for the partial bounds.
+    k_minmax_partial<<<blocks, kMinmaxThreads, 0, s>>>(L, nL, R, ns, step, frac, c->d_scratch);
     SJB_HIP_TRY(hipGetLastError());
-    k_minmax_final<<<1, kMinmaxFinal, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
+    k_minmax_final<<<1, kMinmaxThreads, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }

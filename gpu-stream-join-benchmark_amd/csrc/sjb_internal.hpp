@@ -100,7 +100,7 @@ struct RmiParams {
 
 // Device parameters of range partitioning (written by k_minmax_final).
 struct PartParams {
-    uint32_t kmin;        // min key of both sides
+    uint32_t kmin;        // lower key bound (smallest sampled key of both sides)
     uint32_t shift;       // key bits dropped
 };
 

@@ -123,6 +123,29 @@ def test_smj_sort_key_spans(ctx, torch_mod, orc, spread, size):
         assert np.array_equal(sort_rows(rows), ref_rows)
 
 
+def test_smj_range_bounds_from_sample(ctx, torch_mod, orc):
+    """Range partitioning takes its key bounds from a sample of at most 2^16
+    keys; keys outside them are clamped to the end partitions.  Matching
+    outlier keys (far below and far above the bulk) are put only at
+    positions the sample skips, and must still meet."""
+    nL, nR = 150000, 130000
+    n, ns = nL + nR, 1 << 16
+    sampled = np.zeros(n, dtype=bool)
+    sampled[(np.arange(ns, dtype=np.uint64) * n) // ns] = True
+    free = np.flatnonzero(~sampled)
+    rng = np.random.default_rng(5)
+    R = make(nL, 1, 61, keys=(1000 + rng.integers(0, 1 << 16, nL)).astype(np.uint32))
+    S = make(nR, 1, 62, keys=(1000 + rng.integers(0, 1 << 16, nR)).astype(np.uint32))
+    fl, fr = free[free < nL], free[free >= nL] - nL
+    for key, cl, cr in ((5, 3, 2), (0xFFFFFFF0, 4, 3), (0xFFFFFFFF, 2, 2)):
+        R["key"][rng.choice(fl, cl, replace=False)] = key
+        S["key"][rng.choice(fr, cr, replace=False)] = key
+    o, _ = orc.lazy_hj(R, S, 1 << 12)
+    st = run_count(ctx, torch_mod, SMJ, R, S)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    assert o.count >= 3 * 2 + 4 * 3 + 2 * 2
+
+
 @pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
 def test_capacity_errors(ctx, torch_mod, alg):
     R, S = make(2000, 50, 81), make(2000, 50, 82)
