@@ -852,18 +852,73 @@ __device__ __forceinline__ void sort_tile(uint64_t *s, uint64_t *tmp, int n, boo
     __syncthreads();
 }
 
+// Key span handled by the direct counting sort (one u32 counter per key
+// value, in the scratch of the sort tile).
+constexpr uint32_t kSmjDirect = 4096;
+static_assert(kSmjDirect * sizeof(uint32_t) <= kSmjCap * sizeof(uint64_t), "counters fit the sort scratch");
+static_assert(kSmjDirect % kJoinThreads == 0, "counters split evenly over the threads");
+
+// Sorts the n records of s (n <= kSmjCap) by key when every key lies in
+// [kmin, kmin + kSmjDirect): a counting sort with one counter per key value
+// — one LDS atomic per record (its rank among equal keys), a block scan of
+// the counters, one scattered write per record — instead of a sorting
+// network.  Equal keys come out in arbitrary order (no consumer depends on
+// it).  cnt: kSmjDirect counters, zeroed before the caller's barrier; ws:
+// kJoinThreads / kWave words.  Ends with s sorted and a barrier.
+__device__ __forceinline__ void counting_sort_tile(uint64_t *s, uint32_t *cnt, uint32_t *ws, int n, uint32_t kmin) {
+    constexpr int E = kSmjCap / kJoinThreads;
+    constexpr int P = int(kSmjDirect) / kJoinThreads;
+    static_assert(kSmjDirect <= (1u << 16) && kSmjCap <= (1 << 16), "key offset and rank share a word");
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));  // addresses computed here, not hoisted and held live
+    // only (key offset << 16 | rank) stays live across the scan; the
+    // records are read again for the scatter
+    uint32_t kr[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int i = E * t + r;
+        const uint32_t d = uint32_t(s[i] >> 32) - kmin;
+        kr[r] = i < n ? (d << 16) | atomicAdd(&cnt[d], 1u) : 0u;
+    }
+    __syncthreads();
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < P; q++) sum += cnt[P * t + q];
+    uint32_t tot;
+    uint32_t run = block_excl_scan_u32(sum, ws, &tot);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        const uint32_t c = cnt[P * t + q];
+        cnt[P * t + q] = run;
+        run += c;
+    }
+    __syncthreads();
+    uint64_t x[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) x[r] = s[E * t + r];
+    __syncthreads();  // every record read before the in-place scatter
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int i = E * t + r;
+        if (i < n) s[cnt[kr[r] >> 16] + (kr[r] & 0xFFFFu)] = x[r];
+    }
+    __syncthreads();
+}
+
 // Elements of the sort tile holding n records: 512, 1024 or 2048.
 __device__ __forceinline__ int sort_tile_size(int n) {
     return n <= kJoinThreads ? kJoinThreads : n <= 2 * kJoinThreads ? 2 * kJoinThreads : 4 * kJoinThreads;
 }
 
 // Gathers elements [t0, t0+n) of one side into s as (key << 32 | payload).
-// SORT: pads to the sort tile size with ~0 and sorts (tmp: scratch of the
-// same size; s_mm: 2 * kJoinThreads / kWave words of LDS for the tile's key
-// bounds); padding entries sort last and equal a real (0xFFFFFFFF,
-// 0xFFFFFFFF) record bit for bit, so the first n sorted entries are exactly
-// the tile.  Ends with a barrier.
-template <bool SORT>
+// SORT: pads to the sort tile size with ~0 and sorts (tmp: scratch of
+// kSmjCap 64-bit words; s_mm: 3 * kJoinThreads / kWave words of LDS for the
+// tile's key bounds and scan scratch) — by a counting sort when the keys
+// span less than kSmjDirect values, else by a sorting network; padding
+// entries sort last and equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit
+// for bit, so the first n sorted entries are exactly the tile.  Ends with a
+// barrier.
+template <bool SORT, bool DIRECT = false>
 __device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t *s_mm,
                                             const Rec *__restrict__ part, const PartView &v, int side,
                                             uint32_t t0, int n) {
@@ -887,6 +942,13 @@ __device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t
         }
     }
     if (SORT) {
+        // counters of the direct counting sort (used if the keys' span allows)
+        if (DIRECT) {
+            uint32_t *cnt = reinterpret_cast<uint32_t *>(tmp);
+#pragma unroll
+            for (int q = 0; q < int(kSmjDirect) / kJoinThreads; q++)
+                cnt[q * kJoinThreads + threadIdx.x] = 0;
+        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             mn = min(mn, uint32_t(__shfl_xor(int(mn), off)));
@@ -912,7 +974,9 @@ __device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t
         const bool narrow = __builtin_amdgcn_readfirstlane(mx - mn) < (1u << 20);  // n >= 1: mn <= mx
         const int nu = __builtin_amdgcn_readfirstlane(n);
         const int np2u = __builtin_amdgcn_readfirstlane(np2);
-        if (np2u == kJoinThreads)
+        if (DIRECT && __builtin_amdgcn_readfirstlane(mx - mn) < kSmjDirect)
+            counting_sort_tile(s, reinterpret_cast<uint32_t *>(tmp), s_mm + 2 * kW, nu, mn);
+        else if (np2u == kJoinThreads)
             sort_tile<1>(s, tmp, nu, narrow, mn);
         else if (np2u == 2 * kJoinThreads)
             sort_tile<2>(s, tmp, nu, narrow, mn);
@@ -949,7 +1013,7 @@ template <bool GROUPED>
 struct SmjLds {
     uint64_t b[kSmjCap];                          // sorted build tile
     uint64_t p[kSmjCap];                          // probe tile (unsorted)
-    uint32_t mm[2 * kJoinThreads / kWave];        // per-wave key bounds of a build tile
+    uint32_t mm[3 * kJoinThreads / kWave];        // per-wave key bounds of a build tile + scan scratch
     uint64_t pa[GROUPED ? kSmjCap + 1 : 1];       // grouped: prefix sums of A over the sorted build tile
 };
 
@@ -959,7 +1023,7 @@ struct SmjLds {
 // no sort.  GROUPED: count + C1 from prefix sums of A over the sorted tile;
 // a grouped partition whose build side exceeds one tile (heavy keys) goes
 // to the aggregated build instead, which is O(n) at any skew.
-template <bool C2, bool WRITE, bool GROUPED>
+template <bool C2, bool WRITE, bool GROUPED, bool DIRECT>
 __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &v, SmjLds<GROUPED> &m,
                                               uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
@@ -967,7 +1031,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_tile<true>(s_b, s_p, m.mm, part, v, 0, t0, tn);
+        gather_tile<true, DIRECT>(s_b, s_p, m.mm, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -1069,7 +1133,10 @@ struct JoinLdsBytes {
 };
 
 // Joins partition view v with the body of (ALG, C2, WRITE, GROUPED).
-template <int ALG, bool C2, bool WRITE, bool GROUPED>
+// DIRECT: SMJ build tiles with narrow key spans use the counting sort
+// (the per-window join; the persistent batched join keeps only the
+// sorting networks, whose registers it can hold at three workgroups per CU).
+template <int ALG, bool C2, bool WRITE, bool GROUPED, bool DIRECT>
 __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, int bshift, uint8_t *mem,
                                           uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     if constexpr (ALG == SJB_ALG_HJ) {
@@ -1081,7 +1148,8 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            smj_join_part<C2, WRITE, GROUPED>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base, acc);
+            smj_join_part<C2, WRITE, GROUPED, DIRECT>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base,
+                                                      acc);
     }
 }
 
@@ -1099,7 +1167,10 @@ __host__ __device__ constexpr int idcap_of() {
 // The workgroup's (count, C1, C2) goes to partials[3p ...] and is summed
 // by k_reduce.
 template <int ALG, bool C2, bool WRITE, bool GROUPED>
-__global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t *__restrict__ offs,
+// Count bodies are held to 80 VGPRs (six waves per SIMD): three
+// workgroups per CU, as their LDS allows.
+__global__ __launch_bounds__(kJoinThreads) __attribute__((amdgpu_waves_per_eu(WRITE ? 1 : 6))) void k_join(
+    JoinIO io, const uint32_t *__restrict__ offs,
                                                        int logF, int ncL, int ncR, uint32_t nL,
                                                        int bshift, uint64_t *__restrict__ partials) {
     __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, WRITE>::value];
@@ -1115,7 +1186,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(JoinIO io, const uint32_t
             : load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws);
     Acc acc;
     acc.zero();
-    join_part<ALG, C2, WRITE, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
+    join_part<ALG, C2, WRITE, GROUPED, true>(io, v, bshift, s_mem, s_ws, &s_base, acc);
     // block sum with a single barrier (nothing runs after it)
     acc.count = wave_sum(acc.count);
     acc.c1 = wave_sum(acc.c1);
@@ -1196,7 +1267,7 @@ k_join_batch(BatchArgs a) {
                 Acc acc;
                 acc.zero();
                 const int bshift = ALG == SJB_ALG_HJ ? logF : 0;
-                join_part<ALG, C2, false, GROUPED>(io, v, bshift, s_mem, s_ws, &s_base, acc);
+                join_part<ALG, C2, false, GROUPED, false>(io, v, bshift, s_mem, s_ws, &s_base, acc);
                 const Acc r = block_sum(acc, s_red);
                 if (threadIdx.x == 0 && (r.count | r.c1 | r.c2)) {
                     sjb_window_result_t *o = a.res + bw.id;
