@@ -855,6 +855,10 @@ __device__ __forceinline__ void sort_tile(uint64_t *s, uint64_t *tmp, int n, boo
 // Key span handled by the direct counting sort (one u32 counter per key
 // value, in the scratch of the sort tile).
 constexpr uint32_t kSmjDirect = 4096;
+// Run-id table entries per side of a non-grouped SMJ view (records past them
+// are located by binary search over the runs): 1536 keeps the workgroup,
+// with the key directory, at three per CU; range partitions hold ~1024.
+constexpr int kSmjIdCap = 1536;
 static_assert(kSmjDirect * sizeof(uint32_t) <= kSmjCap * sizeof(uint64_t), "counters fit the sort scratch");
 static_assert(kSmjDirect % kJoinThreads == 0, "counters split evenly over the threads");
 
@@ -863,9 +867,13 @@ static_assert(kSmjDirect % kJoinThreads == 0, "counters split evenly over the th
 // — one LDS atomic per record (its rank among equal keys), a block scan of
 // the counters, one scattered write per record — instead of a sorting
 // network.  Equal keys come out in arbitrary order (no consumer depends on
-// it).  cnt: kSmjDirect counters, zeroed before the caller's barrier; ws:
-// kJoinThreads / kWave words.  Ends with s sorted and a barrier.
-__device__ __forceinline__ void counting_sort_tile(uint64_t *s, uint32_t *cnt, uint32_t *ws, int n, uint32_t kmin) {
+// it).  cnt: kSmjDirect counters, zeroed before the caller's barrier; dir
+// receives the tile's key directory: dir[d] is the first position of key
+// kmin + d, dir[kSmjDirect] = n, so key kmin + d occupies [dir[d],
+// dir[d+1]).  ws: kJoinThreads / kWave words.  Ends with s sorted and a
+// barrier.
+__device__ __forceinline__ void counting_sort_tile(uint64_t *s, uint32_t *cnt, uint16_t *dir, uint32_t *ws, int n,
+                                                   uint32_t kmin) {
     constexpr int E = kSmjCap / kJoinThreads;
     constexpr int P = int(kSmjDirect) / kJoinThreads;
     static_assert(kSmjDirect <= (1u << 16) && kSmjCap <= (1 << 16), "key offset and rank share a word");
@@ -889,9 +897,10 @@ __device__ __forceinline__ void counting_sort_tile(uint64_t *s, uint32_t *cnt, u
 #pragma unroll
     for (int q = 0; q < P; q++) {
         const uint32_t c = cnt[P * t + q];
-        cnt[P * t + q] = run;
+        dir[P * t + q] = uint16_t(run);
         run += c;
     }
+    if (t == 0) dir[kSmjDirect] = uint16_t(n);
     __syncthreads();
     uint64_t x[E];
 #pragma unroll
@@ -900,7 +909,7 @@ __device__ __forceinline__ void counting_sort_tile(uint64_t *s, uint32_t *cnt, u
 #pragma unroll
     for (int r = 0; r < E; r++) {
         const int i = E * t + r;
-        if (i < n) s[cnt[kr[r] >> 16] + (kr[r] & 0xFFFFu)] = x[r];
+        if (i < n) s[dir[kr[r] >> 16] + (kr[r] & 0xFFFFu)] = x[r];
     }
     __syncthreads();
 }
@@ -918,10 +927,17 @@ __device__ __forceinline__ int sort_tile_size(int n) {
 // entries sort last and equal a real (0xFFFFFFFF, 0xFFFFFFFF) record bit
 // for bit, so the first n sorted entries are exactly the tile.  Ends with a
 // barrier.
+// How a build tile was sorted: direct = counting sort, whose key directory
+// (relative to kmin) then answers probes.
+struct TileSort {
+    bool direct;
+    uint32_t kmin;
+};
+
 template <bool SORT, bool DIRECT = false>
-__device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t *s_mm,
-                                            const Rec *__restrict__ part, const PartView &v, int side,
-                                            uint32_t t0, int n) {
+__device__ __forceinline__ TileSort gather_tile(uint64_t *s, uint64_t *tmp, uint32_t *s_mm, uint16_t *dir,
+                                                const Rec *__restrict__ part, const PartView &v, int side,
+                                                uint32_t t0, int n) {
     const int np2 = SORT ? sort_tile_size(n) : n;
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
     for (int i0 = 0; i0 < np2; i0 += kJoinThreads * kBatch) {
@@ -974,15 +990,18 @@ __device__ __forceinline__ void gather_tile(uint64_t *s, uint64_t *tmp, uint32_t
         const bool narrow = __builtin_amdgcn_readfirstlane(mx - mn) < (1u << 20);  // n >= 1: mn <= mx
         const int nu = __builtin_amdgcn_readfirstlane(n);
         const int np2u = __builtin_amdgcn_readfirstlane(np2);
-        if (DIRECT && __builtin_amdgcn_readfirstlane(mx - mn) < kSmjDirect)
-            counting_sort_tile(s, reinterpret_cast<uint32_t *>(tmp), s_mm + 2 * kW, nu, mn);
-        else if (np2u == kJoinThreads)
+        if (DIRECT && __builtin_amdgcn_readfirstlane(mx - mn) < kSmjDirect) {
+            counting_sort_tile(s, reinterpret_cast<uint32_t *>(tmp), dir, s_mm + 2 * kW, nu, mn);
+            return TileSort{true, mn};
+        }
+        if (np2u == kJoinThreads)
             sort_tile<1>(s, tmp, nu, narrow, mn);
         else if (np2u == 2 * kJoinThreads)
             sort_tile<2>(s, tmp, nu, narrow, mn);
         else
             sort_tile<4>(s, tmp, nu, narrow, mn);
     }
+    return TileSort{false, 0u};
 }
 
 __device__ __forceinline__ int lower_bound_key(const uint64_t *s, int n, uint32_t k) {
@@ -1014,6 +1033,7 @@ struct SmjLds {
     uint64_t b[kSmjCap];                          // sorted build tile
     uint64_t p[kSmjCap];                          // probe tile (unsorted)
     uint32_t mm[3 * kJoinThreads / kWave];        // per-wave key bounds of a build tile + scan scratch
+    uint16_t dir[kSmjDirect + 2];                 // counting-sorted tile: first position of each key
     uint64_t pa[GROUPED ? kSmjCap + 1 : 1];       // grouped: prefix sums of A over the sorted build tile
 };
 
@@ -1031,7 +1051,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        gather_tile<true, DIRECT>(s_b, s_p, m.mm, part, v, 0, t0, tn);
+        const TileSort ts = gather_tile<true, DIRECT>(s_b, s_p, m.mm, m.dir, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -1068,12 +1088,20 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
         }
         for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
             const int un = int(min<uint32_t>(kSmjCap, np - u0));
-            gather_tile<false>(s_p, nullptr, nullptr, part, v, 1, u0, un);
+            gather_tile<false>(s_p, nullptr, nullptr, nullptr, part, v, 1, u0, un);
             if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
-                    const int lo = lower_bound_key(s_b, tn, k);
-                    const int hi = upper_bound_key(s_b, lo, tn, k);
+                    int lo, hi;
+                    if (DIRECT && ts.direct) {  // the counting sort's key directory
+                        const uint32_t d = k - ts.kmin;
+                        if (d >= kSmjDirect) continue;
+                        lo = m.dir[d];
+                        hi = m.dir[d + 1];
+                    } else {
+                        lo = lower_bound_key(s_b, tn, k);
+                        hi = upper_bound_key(s_b, lo, tn, k);
+                    }
                     if (hi == lo) continue;
                     if constexpr (GROUPED) {
                         acc.count += uint64_t(hi - lo);
@@ -1158,7 +1186,7 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
 // workgroup fits three to a CU.
 template <int ALG, bool GROUPED>
 __host__ __device__ constexpr int idcap_of() {
-    return ALG == SJB_ALG_SMJ ? kSmjCap : (GROUPED ? kAggIdCap : kHjCap);
+    return ALG == SJB_ALG_SMJ ? (GROUPED ? kSmjCap : kSmjIdCap) : (GROUPED ? kAggIdCap : kHjCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1167,9 +1195,10 @@ __host__ __device__ constexpr int idcap_of() {
 // The workgroup's (count, C1, C2) goes to partials[3p ...] and is summed
 // by k_reduce.
 template <int ALG, bool C2, bool WRITE, bool GROUPED>
-// Count bodies are held to 80 VGPRs (six waves per SIMD): three
-// workgroups per CU, as their LDS allows.
-__global__ __launch_bounds__(kJoinThreads) __attribute__((amdgpu_waves_per_eu(WRITE ? 1 : 6))) void k_join(
+// Count bodies whose LDS allows three workgroups per CU are held to 80
+// VGPRs (six waves per SIMD) to match.
+__global__ __launch_bounds__(kJoinThreads)
+__attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 : 6))) void k_join(
     JoinIO io, const uint32_t *__restrict__ offs,
                                                        int logF, int ncL, int ncR, uint32_t nL,
                                                        int bshift, uint64_t *__restrict__ partials) {
@@ -1333,7 +1362,7 @@ int launch_reduce(sjb_ctx *c, int n_partials, sjb_window_result_t *d_out, uint32
 }
 
 int join_idcap(int alg, bool grouped) {
-    return alg == SJB_ALG_SMJ ? kSmjCap : (grouped ? kAggIdCap : kHjCap);
+    return alg == SJB_ALG_SMJ ? (grouped ? kSmjCap : kSmjIdCap) : (grouped ? kAggIdCap : kHjCap);
 }
 
 size_t join_view_bytes(int alg, bool grouped, int ncL, int ncR) {
