@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 128 MiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", default="window", choices=["window", "batch", "learned", "slide", "nlj"],
+    ap.add_argument("--workload", default="window", choices=["window", "batch", "learned", "slide", "nlj", "eager"],
                     help="window: 2^22-tuple tumbling windows (default); batch: 4096 lognormal "
                          "windows per launch, Zipf keys over [0, 2^28)")
     ap.add_argument("--theta", type=float, default=0.0, help="batch: Zipf theta (0 = uniform)")
@@ -319,6 +319,97 @@ def bench_nlj(a):
     print(json.dumps(out))
 
 
+def bench_eager(a):
+    """Eager (per-batch) joins on host batches, as the eager executors
+    receive them: each batch is copied to the device, probes the other
+    side's resident window state and is appended to its own side; the
+    window (W tuples per side) closes when both sides are full.  Keys
+    uniform in [0, 2^20).  The persistent-kernel HJ serves each batch with
+    one resident workgroup (no launches per batch).  Per configuration: input tuples/s over all
+    batches and p50/p99 of the per-batch call latency (host call to result,
+    the H2D copy included).  Count + C1 + C2 per batch."""
+    import numpy as np
+    import sjb
+
+    cases = [("HJ", 1 << 20, 4, 1 << 14), ("PK_HJ", 1 << 20, 4, 1 << 14), ("NLJ", 1 << 17, 4, 1 << 14),
+             ("HJ_small_batches", 1 << 16, 3, 1 << 10), ("PK_HJ_small_batches", 1 << 16, 3, 1 << 10)]
+    rows = {}
+    for name, W, nwin, B in cases:
+        nb = W // B
+        R = sjb.generate(W * nwin, sjb.DIST_UNIFORM, 1 << 20, 0.0, seed=123)
+        S = sjb.generate(W * nwin, sjb.DIST_UNIFORM, 1 << 20, 0.0, seed=456)
+        ctx = sjb.Context(device=0, max_window=W, distinct_keys=1 << 20, batch_size=B)
+        pk = name.startswith("PK_HJ")
+        alg = sjb.ALG["NLJ" if name == "NLJ" else "HJ"]
+        lat, matches = [], 0
+
+        def run(w, timed):
+            nonlocal matches
+            for b in range(nb):
+                for is_left, src in ((1, R), (0, S)):
+                    batch = src[(w * nb + b) * B:(w * nb + b + 1) * B]
+                    t0 = time.perf_counter()
+                    if pk:
+                        rc, st = ctx.pk_execute(batch, is_left)
+                    else:
+                        rc, st = ctx.eager_execute(alg, sjb.OUT["NoOutput"], sjb.F_C2, batch, is_left)
+                    if rc != 0:
+                        raise SystemExit(f"eager {name}: rc {rc}")
+                    if timed:
+                        lat.append(time.perf_counter() - t0)
+                        matches += st.result_count
+            if pk:
+                ctx.pk_clear()
+            else:
+                ctx.eager_clear()
+
+        run(0, False)  # warm-up window
+        t_start = time.perf_counter()
+        for w in range(1, nwin):
+            run(w, True)
+        el = time.perf_counter() - t_start
+        if pk:
+            ctx.pk_stop()
+        ctx.close()
+        lat_ms = np.array(lat) * 1e3
+        rows[name] = {"window_per_side": W, "batch": B, "windows_timed": nwin - 1,
+                      "tuples_per_s": 2 * W * (nwin - 1) / el,
+                      "p50_batch_ms": float(np.percentile(lat_ms, 50)),
+                      "p99_batch_ms": float(np.percentile(lat_ms, 99)), "matches": int(matches)}
+        print(f"[bench] eager {name}: {rows[name]}", file=sys.stderr, flush=True)
+    hj = rows["HJ"]
+    out = {
+        "metric": "eager per-batch join: input tuples/s and p50/p99 per-batch latency (host batches)",
+        "value": hj["tuples_per_s"], "unit": "tuples/s", "n_gpus": 1,
+        "steps": hj["windows_timed"], "warmup": 1,
+        "ms_per_step": 1e3 * 2 * hj["window_per_side"] / hj["tuples_per_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 keys / u64 sums", "data": "synthetic (splitmix64 generator, uniform keys in [0, 2^20))",
+        "config": {"workload": "eager tumbling windows, 2^14-tuple host batches alternating sides; HJ and "
+                               "persistent-kernel HJ at 2^20 tuples/side, NLJ at 2^17; also HJ and "
+                               "persistent-kernel HJ with 2^10-tuple batches at 2^16 tuples/side",
+                   "checksums": "count+C1+C2", "parallelism": "single GPU"},
+        "cases": rows,
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        W = 1 << 20
+        R = sjb.generate(W, sjb.DIST_UNIFORM, 1 << 20, 0.0, seed=123)
+        S = sjb.generate(W, sjb.DIST_UNIFORM, 1 << 20, 0.0, seed=456)
+        e = oracle.Eager(1 << 20, W)
+        B = 1 << 14
+        t0 = time.perf_counter()
+        for b in range(W // B):
+            e.execute(R[b * B:(b + 1) * B], True)
+            e.execute(S[b * B:(b + 1) * B], False)
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 2 * W / secs, "unit": "tuples/s", "cores": 1, "kind": "port",
+                               "sample": "one window of 2^20/side in 2^14-tuple batches, eager CPU_HJ "
+                                         "restated in C, one thread", "seconds": secs}
+    print(json.dumps(out))
+
+
 def bench_batch(a):
     """4096 windows per launch, per-side sizes LogNormal(median 2^14, sigma 1)
     clipped to [2^8, 2^20], keys Zipf(theta) over [0, 2^28) (uniform at
@@ -416,6 +507,8 @@ def main():
         return bench_slide(a)
     if a.workload == "nlj":
         return bench_nlj(a)
+    if a.workload == "eager":
+        return bench_eager(a)
     import numpy as np
     import torch
     import sjb

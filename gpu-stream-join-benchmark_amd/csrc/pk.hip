@@ -31,6 +31,10 @@
 namespace sjb {
 
 constexpr int kPkThreads = 1024;
+#ifndef SJB_PK_ILP
+#define SJB_PK_ILP 6
+#endif
+constexpr int kPkIlp = SJB_PK_ILP;  // batch tuples per thread whose memory chains are in flight together
 constexpr int kPkQ = 8;  // descriptor / batch-slot ring
 constexpr uint32_t kPkNil = 0xFFFFFFFFu;
 
@@ -106,30 +110,70 @@ __global__ __launch_bounds__(kPkThreads) void k_pk_eager_hj(
             const sjb_tuple_t *otup = other ? tup1 : tup0;
             const uint32_t *onext = other ? next1 : next0;
             const uint32_t *ohead = other ? head1 : head0;
-            // probe the opposite side
-            for (uint64_t i = threadIdx.x; i < d.n; i += kPkThreads) {
-                const sjb_tuple_t t = batch[i];
-                for (uint32_t g = ohead[key_hash(t.key) & nb_mask]; g != kPkNil; g = onext[g]) {
-                    const sjb_tuple_t o = otup[g];
-                    if (o.key != t.key) continue;
-                    const sjb_tuple_t &l = own == 0 ? t : o;
-                    const sjb_tuple_t &r = own == 0 ? o : t;
-                    const uint64_t a = hash_a(l.key, l.val);
-                    acc.count++;
-                    acc.c1 += a * hash_b(r.key, r.val);
-                    if (c2) acc.c2 += pair_hash_from_a(a, r.val);
+            // probe the opposite side: kPkIlp tuples per thread at a time,
+            // their chains walked in lockstep (all loads of a step in flight);
+            // key and value are all the checksums need (8-byte loads)
+            for (uint64_t i0 = 0; i0 < d.n; i0 += uint64_t(kPkThreads) * kPkIlp) {
+                uint2 t[kPkIlp];
+                uint32_t g[kPkIlp];
+#pragma unroll
+                for (int j = 0; j < kPkIlp; j++) {
+                    const uint64_t i = i0 + uint64_t(j) * kPkThreads + threadIdx.x;
+                    t[j] = i < d.n ? *reinterpret_cast<const uint2 *>(&batch[i]) : make_uint2(0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < kPkIlp; j++) {
+                    const uint64_t i = i0 + uint64_t(j) * kPkThreads + threadIdx.x;
+                    g[j] = i < d.n ? ohead[key_hash(t[j].x) & nb_mask] : kPkNil;
+                }
+                for (;;) {
+                    bool any = false;
+                    uint2 o[kPkIlp];
+                    uint32_t nx[kPkIlp];
+#pragma unroll
+                    for (int j = 0; j < kPkIlp; j++) {
+                        if (g[j] != kPkNil) {
+                            o[j] = *reinterpret_cast<const uint2 *>(&otup[g[j]]);
+                            nx[j] = onext[g[j]];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < kPkIlp; j++) {
+                        if (g[j] == kPkNil) continue;
+                        if (o[j].x == t[j].x) {
+                            const uint32_t lv = own == 0 ? t[j].y : o[j].y, rv = own == 0 ? o[j].y : t[j].y;
+                            const uint64_t a = hash_a(t[j].x, lv);
+                            acc.count++;
+                            acc.c1 += a * hash_b(t[j].x, rv);
+                            if (c2) acc.c2 += pair_hash_from_a(a, rv);
+                        }
+                        g[j] = nx[j];
+                        any |= g[j] != kPkNil;
+                    }
+                    if (!any) break;
                 }
             }
-            // append to the own side
+            // append to the own side (kPkIlp tuples per thread at a time)
             sjb_tuple_t *tt = own ? tup1 : tup0;
             uint32_t *tn = own ? next1 : next0;
             uint32_t *th = own ? head1 : head0;
             const uint64_t base = s_n[own];
-            for (uint64_t i = threadIdx.x; i < d.n; i += kPkThreads) {
-                const sjb_tuple_t t = batch[i];
-                const uint32_t g = uint32_t(base + i);
-                tt[g] = t;
-                tn[g] = atomicExch(&th[key_hash(t.key) & nb_mask], g);
+            for (uint64_t i0 = 0; i0 < d.n; i0 += uint64_t(kPkThreads) * kPkIlp) {
+                sjb_tuple_t t[kPkIlp];
+#pragma unroll
+                for (int j = 0; j < kPkIlp; j++) {
+                    const uint64_t i = i0 + uint64_t(j) * kPkThreads + threadIdx.x;
+                    if (i < d.n) t[j] = batch[i];
+                }
+#pragma unroll
+                for (int j = 0; j < kPkIlp; j++) {
+                    const uint64_t i = i0 + uint64_t(j) * kPkThreads + threadIdx.x;
+                    if (i < d.n) {
+                        const uint32_t gg = uint32_t(base + i);
+                        tt[gg] = t[j];
+                        tn[gg] = atomicExch(&th[key_hash(t[j].key) & nb_mask], gg);
+                    }
+                }
             }
             __syncthreads();
             if (threadIdx.x == 0) {
