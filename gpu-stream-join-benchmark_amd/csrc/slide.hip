@@ -23,8 +23,12 @@
 //   - J(L_keep, R_out) + J(L_in, R_keep) + J(L_new, R_in).
 // Tuples of each side also live in a device ring of W slots (the expiring
 // ones are read back from it).  Removing a key leaves a slot with count 0;
-// tables have 4W slots, and when slots in use could pass half of them the
-// table is rebuilt from the ring (a rebuilt table holds at most W keys).
+// tables have SJB_SLIDE_SLOTS * W slots (8W: a rebuild, ~1 ms at W = 2^22,
+// comes rarely enough to stay out of the 99th latency percentile — slots
+// used since a rebuild grow with the distinct keys seen, ever more slowly
+// under skew; with 4W, rebuilds every ~200 events put p99 at 1.2 ms), and
+// when slots in use could pass half of them the table is rebuilt from the
+// ring (a rebuilt table holds at most W keys).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -114,6 +118,11 @@ __device__ __forceinline__ uint32_t slide_find_insert_from(SlideSlot *__restrict
 // Tuples (threads) per step workgroup: larger groups combine more updates
 // of hot keys, smaller ones spread a batch over more CUs; 512 measured
 // best for batches of 2^16 Zipf(1.0) tuples (1024: -10%, 256: -6%).
+#ifndef SJB_SLIDE_SLOTS
+#define SJB_SLIDE_SLOTS 8  // table slots per window tuple (a power of two)
+#endif
+static_assert(SJB_SLIDE_SLOTS >= 2 && SJB_SLIDE_SLOTS <= 8 && (SJB_SLIDE_SLOTS & (SJB_SLIDE_SLOTS - 1)) == 0,
+              "slot indices stay 32-bit for windows up to 2^28");
 #ifndef SJB_SLIDE_THREADS
 #define SJB_SLIDE_THREADS 512
 #endif
@@ -432,7 +441,7 @@ int sjb_slide_create(sjb_ctx_t *c, uint64_t window, uint64_t max_batch, sjb_slid
     *out = nullptr;
     SJB_HIP_TRY(hipSetDevice(c->device));
     int log_cap = 10;
-    while ((1ull << log_cap) < 4 * window) log_cap++;
+    while ((1ull << log_cap) < uint64_t(SJB_SLIDE_SLOTS) * window) log_cap++;
     const uint64_t slots = (1ull << log_cap) + 1;
     const uint64_t bytes = 2 * (window * sizeof(sjb_tuple_t) + slots * sizeof(SlideSlot)) +
                            4 * max_batch * sizeof(sjb_tuple_t) + 4096;
