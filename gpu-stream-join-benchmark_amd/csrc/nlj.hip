@@ -76,15 +76,8 @@ __device__ __forceinline__ void nlj_tile_hits(const uint32_t *__restrict__ tk, i
     }
 }
 
-#ifndef SJB_NLJ_WPE
-#define SJB_NLJ_WPE 0
-#endif
 template <bool C2, bool WRITE>
-__global__ __launch_bounds__(kNljThreads)
-#if SJB_NLJ_WPE
-__attribute__((amdgpu_waves_per_eu(WRITE ? 1 : SJB_NLJ_WPE)))
-#endif
-void k_nlj(
+__global__ __launch_bounds__(kNljThreads) void k_nlj(
     const uint32_t *__restrict__ lkeys, const sjb_tuple_t *__restrict__ L, uint64_t nL,
     const sjb_tuple_t *__restrict__ R, uint64_t nR, uint64_t l_split, uint64_t *__restrict__ partials,
     sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
