@@ -55,7 +55,7 @@ constexpr int kHjPerP = kHjProbe / kJoinThreads;        // staged probe records 
 
 // Record index of element j (< pref[nc]) by binary search over the run
 // prefix (used only past the run-id table).
-__device__ __forceinline__ uint32_t run_index(const uint32_t *pref, const uint32_t *start, int nc,
+__device__ __forceinline__ uint32_t run_index(const uint32_t *pref, const uint32_t *base, int nc,
                                               uint32_t j) {
     int lo = 0, hi = nc;
     while (hi - lo > 1) {
@@ -65,15 +65,16 @@ __device__ __forceinline__ uint32_t run_index(const uint32_t *pref, const uint32
         else
             hi = mid;
     }
-    return start[lo] + (j - pref[lo]);
+    return base[lo] + j;
 }
 
 // One partition of a window as seen from its join workgroup: for each side
-// the start of every chunk's run in d_part, the exclusive prefix of the run
-// lengths, and a run-id table (element j -> run) for the first `idcap`
-// elements, so a record index costs three LDS reads.
+// the exclusive prefix pref[c] of the run lengths, base[c] = (start of chunk
+// c's run in d_part) - pref[c] (mod 2^32), so element j of run c is record
+// base[c] + j, and a run-id table (element j -> run) for the first `idcap`
+// elements: a record index costs two LDS reads.
 struct PartView {
-    uint32_t *start[2];
+    uint32_t *base[2];
     uint32_t *pref[2];
     uint16_t *runid[2];
     int nc[2];
@@ -82,13 +83,13 @@ struct PartView {
     __device__ __forceinline__ uint32_t index(int side, uint32_t j) const {
         if (j < idcap) {
             const uint32_t c = runid[side][j];
-            return start[side][c] + (j - pref[side][c]);
+            return base[side][c] + j;
         }
-        return run_index(pref[side], start[side], nc[side], j);
+        return run_index(pref[side], base[side], nc[side], j);
     }
 };
 
-// Dynamic LDS bytes of a PartView: start/pref for both sides plus two
+// Dynamic LDS bytes of a PartView: base/pref for both sides plus two
 // run-id tables of idcap entries.
 __host__ __device__ constexpr size_t part_view_bytes(int ncL, int ncR, int idcap) {
     return size_t(4) * (size_t(ncL) + ncR + (ncL + 1) + (ncR + 1)) + size_t(2) * 2 * idcap + 16;
@@ -105,8 +106,8 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     PartView v;
     v.nc[0] = ncL;
     v.nc[1] = ncR;
-    v.start[0] = dyn;
-    v.start[1] = dyn + ncL;
+    v.base[0] = dyn;
+    v.base[1] = dyn + ncL;
     v.pref[0] = dyn + ncL + ncR;
     v.pref[1] = v.pref[0] + ncL + 1;
     v.runid[0] = reinterpret_cast<uint16_t *>(v.pref[1] + ncR + 1);
@@ -125,7 +126,7 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
             const int c = right ? q - ncL : q;
             const uint32_t *o = offs + uint64_t(q) * F1;
             const uint32_t a = o[p], b = o[p + 1];
-            v.start[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a;
+            v.base[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a;  // run start for now
             len = b - a;
         }
         uint32_t tot;
@@ -152,11 +153,12 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     v.n[1] = carry - nleft;
     __syncthreads();
     // run-id fill: one thread per run of either side (slot q as above), so
-    // both sides fill at once
+    // both sides fill at once; the run start becomes base = start - pref
     for (int q = threadIdx.x; q < nq; q += blockDim.x) {
         const int side = q >= ncL;
         const int c = side ? q - ncL : q;
         const uint32_t a = v.pref[side][c];
+        v.base[side][c] -= a;
         const uint32_t b = min(v.pref[side][c + 1], idcap);
         for (uint32_t j = a; j < b; j++) v.runid[side][j] = uint16_t(c);
     }
@@ -174,8 +176,8 @@ __device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_
     PartView v;
     v.nc[0] = ncL;
     v.nc[1] = ncR;
-    v.start[0] = dyn;
-    v.start[1] = dyn + ncL;
+    v.base[0] = dyn;
+    v.base[1] = dyn + ncL;
     v.pref[0] = dyn + ncL + ncR;
     v.pref[1] = v.pref[0] + ncL + 1;
     v.runid[0] = reinterpret_cast<uint16_t *>(v.pref[1] + ncR + 1);
@@ -205,8 +207,9 @@ __device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_
         tot += x;
     }
     if (q < nq) {
-        v.start[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a;
-        v.pref[right][c] = pre + ex - (right ? nleft : 0u);
+        const uint32_t pf = pre + ex - (right ? nleft : 0u);
+        v.base[right][c] = (right ? rbase : lbase) + uint32_t(c) * chunk + a - pf;
+        v.pref[right][c] = pf;
     }
     if (threadIdx.x == 0) {
         v.pref[0][ncL] = nleft;
