@@ -369,12 +369,54 @@ __device__ __forceinline__ void pairq_drain(const PairQ &q, uint32_t n, Acc &acc
 // Count / C1 / C2 of up to kHjPerP register-resident probe records per
 // thread (element j = k * blockDim + tid < un) against the current tile
 // (payload = val).  The bucket scan compares keys only; the first two
-// matches of every probe go to the wave's queue (later matches, rare, are
-// summed in place).
-template <bool C2>
+// matches of every probe go to the wave's queue (later matches, rare under
+// uniform keys, are summed in place).  ALLQ (the batched join, built for
+// skewed windows): every match goes through the queue instead — fewer
+// idle lanes under hot keys, ~5% more work under uniform keys.
+template <bool C2, bool ALLQ>
 __device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[kHjPerP], uint32_t un,
                                               int bshift, Acc &acc, PairQ &q) {
     uint32_t qn = 0;  // queued matches (wave-uniform)
+    if constexpr (ALLQ) {
+        // every match through the queue: per round each lane advances its
+        // bucket scan to its next match and the wave queues the round's
+        // matches, so a hot key's long match list runs on full waves
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kHjPerP; k++) {
+            const uint32_t j = k * kJoinThreads + threadIdx.x;
+            const Rec s = rp[k];
+            uint32_t e = 0, end = 0;
+            if (j < un) {
+                const uint32_t b = hj_bucket(s.x, bshift);
+                e = t.off[b];
+                end = t.off[b + 1];
+            }
+            for (;;) {
+                while (e < end && t.key[e] != s.x) e++;
+                const bool has = e < end;
+                const uint64_t mask = __ballot(has);
+                if (mask == 0) break;  // wave-uniform
+                const uint32_t c = uint32_t(__popcll(mask));
+                if (qn + c > uint32_t(kWave)) {
+                    pairq_drain<C2>(q, qn, acc);
+                    qn = 0;
+                }
+                if (has) {
+                    const uint32_t pos = qn + lanes_below(mask);
+                    q.key[pos] = s.x;
+                    q.sv[pos] = s.y;
+                    q.rv[pos] = t.pay[e];
+                    cnt++;
+                    e++;
+                }
+                qn += c;
+            }
+        }
+        acc.count += cnt;
+        pairq_drain<C2>(q, qn, acc);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kHjPerP; k++) {
         const uint32_t j = k * kJoinThreads + threadIdx.x;
@@ -491,7 +533,7 @@ struct HjLds<false> {
 // or, when pair rows are written, go to an LDS staging buffer); larger
 // partitions continue tile by tile.  The caller has built `v` (load_view
 // ends with a barrier).
-template <bool C2, bool WRITE>
+template <bool C2, bool WRITE, bool ALLQ>
 __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v, int bshift,
                                              HjLds<WRITE> &m, uint32_t *s_ws, unsigned long long *s_base,
                                              Acc &acc) {
@@ -554,7 +596,7 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
                     }
                     resident = u0;
                 }
-                hj_probe_regs<C2>(t, rp, un, bshift, acc, m.q[threadIdx.x / kWave]);
+                hj_probe_regs<C2, ALLQ>(t, rp, un, bshift, acc, m.q[threadIdx.x / kWave]);
             }
         }
         __syncthreads();
@@ -1164,22 +1206,24 @@ struct JoinLdsBytes {
 };
 
 // Joins partition view v with the body of (ALG, C2, WRITE, GROUPED).
-// DIRECT: SMJ build tiles with narrow key spans use the counting sort
-// (the per-window join; the persistent batched join keeps only the
-// sorting networks, whose registers it can hold at three workgroups per CU).
-template <int ALG, bool C2, bool WRITE, bool GROUPED, bool DIRECT>
+// PER_WINDOW: the per-window join (k_join) — SMJ build tiles with narrow key
+// spans use the counting sort (the persistent batched join keeps only the
+// sorting networks, whose registers it can hold at three workgroups per CU);
+// the batched join's HJ probe queues every match (its windows are skewed).
+template <int ALG, bool C2, bool WRITE, bool GROUPED, bool PER_WINDOW>
 __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, int bshift, uint8_t *mem,
                                           uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     if constexpr (ALG == SJB_ALG_HJ) {
         if constexpr (GROUPED)
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            hj_join_part<C2, WRITE>(io, v, bshift, *reinterpret_cast<HjLds<WRITE> *>(mem), s_ws, s_base, acc);
+            hj_join_part<C2, WRITE, !PER_WINDOW>(io, v, bshift, *reinterpret_cast<HjLds<WRITE> *>(mem), s_ws,
+                                                 s_base, acc);
     } else {
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            smj_join_part<C2, WRITE, GROUPED, DIRECT>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base,
+            smj_join_part<C2, WRITE, GROUPED, PER_WINDOW>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base,
                                                       acc);
     }
 }
