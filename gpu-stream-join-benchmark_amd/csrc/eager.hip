@@ -32,10 +32,32 @@ __device__ __forceinline__ uint32_t eager_bucket(uint32_t k, uint32_t nb_mask) {
     return key_hash(k) & nb_mask;
 }
 
+// Appends the batch to its own side.  Workgroup 0 also sums the probe's
+// n_partials per-block (count, C1, C2) partials into *res (the batch's
+// result), so no separate reduction launch follows the probe.
 __global__ __launch_bounds__(256) void k_eager_build(const sjb_tuple_t *__restrict__ batch, uint64_t n,
                                                      uint64_t base, sjb_tuple_t *__restrict__ tuples,
                                                      uint32_t *__restrict__ next,
-                                                     uint32_t *__restrict__ head, uint32_t nb_mask) {
+                                                     uint32_t *__restrict__ head, uint32_t nb_mask,
+                                                     const uint64_t *__restrict__ partials, uint32_t n_partials,
+                                                     sjb_window_result_t *__restrict__ res) {
+    __shared__ uint64_t s_red[3 * 256 / kWave];
+    if (blockIdx.x == 0 && n_partials) {  // uniform per workgroup
+        Acc a;
+        a.zero();
+        for (uint32_t b = threadIdx.x; b < n_partials; b += blockDim.x) {
+            a.count += partials[3 * b];
+            a.c1 += partials[3 * b + 1];
+            a.c2 += partials[3 * b + 2];
+        }
+        const Acc r = block_sum(a, s_red);
+        if (threadIdx.x == 0) {
+            res->count = r.count;
+            res->checksum1 = r.c1;
+            res->checksum2 = r.c2;
+            res->pairs = 0;
+        }
+    }
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const sjb_tuple_t t = batch[i];
@@ -198,6 +220,7 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
     }
     const uint64_t n_other = c->eager_n[other];
     sjb_window_result_t res{};
+    uint32_t probe_partials = 0;
     if (write && !wrap && n_other) {
         // Count first so an undersized output buffer is rejected before the
         // state changes (the caller can grow the buffer and call again).
@@ -243,8 +266,7 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
             SJB_EP(false, false);
 #undef SJB_EP
         SJB_HIP_TRY(hipGetLastError());
-        rc = launch_reduce(c, int(blocks), c->d_result, 0);
-        if (rc) return rc;
+        probe_partials = blocks;  // summed by the append kernel
     } else {
         // NLJ: incoming batch x the other side's resident window
         // (eager countKernel / getJoinResultBitmapEager).  The reference
@@ -262,11 +284,12 @@ int sjb_eager_execute(sjb_ctx_t *c, int algorithm, int method, uint32_t flags,
         }
         if (rc) return rc;
     }
-    // append the batch to its own side
+    // append the batch to its own side (and, after an HJ probe, sum its
+    // partials into the result)
     if (algorithm == SJB_ALG_HJ) {
         k_eager_build<<<unsigned((n + 255) / 256), 256, 0, c->stream>>>(
             c->d_batch, n, c->eager_n[own], c->d_eager_tuples[own], c->d_eager_next[own],
-            c->d_eager_head[own], uint32_t(c->eager_nb - 1));
+            c->d_eager_head[own], uint32_t(c->eager_nb - 1), c->d_partials, probe_partials, c->d_result);
         SJB_HIP_TRY(hipGetLastError());
     } else {
         SJB_HIP_TRY(hipMemcpyAsync(c->d_eager_tuples[own] + c->eager_n[own], c->d_batch,
