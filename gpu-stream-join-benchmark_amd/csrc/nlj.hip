@@ -110,16 +110,27 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
         const uint32_t *__restrict__ tk = lkeys + t0;
         uint32_t m = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu;
         const int t16 = tn & ~31;
-        for (int e = 0; e < t16; e += 32) {
+        // software-pipelined key loads: the next 16 keys are requested
+        // before the current 16 are compared (the reads past t16 stay in
+        // the key array's slack and are not compared)
+        uint32_t kc[16], kn[16];
 #pragma unroll
-            for (int c = 0; c < 32; c++) {
-                const uint32_t lk = tk[e + c];
+        for (int c = 0; c < 16; c++) kc[c] = tk[c];
+        for (int e = 0; e < t16; e += 16) {
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): kc has landed
+#pragma unroll
+            for (int c = 0; c < 16; c++) kn[c] = tk[e + 16 + c];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                const uint32_t lk = kc[c];
 #pragma unroll
                 for (int i = 0; i < kNljPer; i += 4) {
                     m = min(min(m, lk ^ sk[i]), lk ^ sk[i + 1]);
                     m2 = min(min(m2, lk ^ sk[i + 2]), lk ^ sk[i + 3]);
                 }
             }
+#pragma unroll
+            for (int c = 0; c < 16; c++) kc[c] = kn[c];
         }
         for (int e = t16; e < tn; e++) {
             const uint32_t lk = tk[e];
