@@ -10,6 +10,8 @@ Prints ONE JSON line (rank 0):
   value        joined input tuples/s = 2 * 2^26 * steps / time
   roofline     HBM roofline of the dominant kernel (algorithmic bytes per
                launch / average launch time, HIP events on the ctx stream)
+  copy_GBps    a plain 1 GiB device copy's rate (read + write bytes), the
+               achievable streaming bandwidth; phases[*].frac_of_copy
   window_ms    p50 / p99 device time per window
   cpu_baseline the C restatement of the reference CPU hash join (oracle),
                timed on a bounded sample of the same workload
@@ -632,6 +634,22 @@ def main():
                               "frac": pipe_bytes / avg_win / HBM_PEAK},
         "result_window0": {"count": int(last[0]["count"]), "c1": hex(int(last[0]["checksum1"]))},
     }
+    # achievable streaming rate for reference: a plain 1 GiB device copy
+    # (read + write), timed with events after the measured steps
+    cbuf = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+    cdst = torch.empty_like(cbuf)
+    cdst.copy_(cbuf)
+    ce = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ce[0].record()
+    for _ in range(5):
+        cdst.copy_(cbuf)
+    ce[1].record()
+    torch.cuda.synchronize()
+    copy_bps = 5 * 2 * (1 << 30) / (ce[0].elapsed_time(ce[1]) / 1e3)
+    del cbuf, cdst
+    for v in phases.values():
+        v["frac_of_copy"] = v["achieved_GBps"] * 1e9 / copy_bps
+    out["copy_GBps"] = copy_bps / 1e9
     if a.calibrate_copy:
         src = torch.empty(128 << 20, dtype=torch.uint8, device="cuda").fill_(1)
         dst = torch.empty_like(src)
