@@ -1088,7 +1088,7 @@ struct SmjLds {
 // no sort.  GROUPED: count + C1 from prefix sums of A over the sorted tile;
 // a grouped partition whose build side exceeds one tile (heavy keys) goes
 // to the aggregated build instead, which is O(n) at any skew.
-template <bool C2, bool WRITE, bool GROUPED, bool DIRECT>
+template <bool C2, bool WRITE, bool GROUPED, bool PER_WINDOW, bool EXPAND>
 __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &v, SmjLds<GROUPED> &m,
                                               uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     const uint32_t nb = v.n[0], np = v.n[1];
@@ -1096,7 +1096,7 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
     uint64_t *s_b = m.b, *s_p = m.p, *s_pa = m.pa;
     for (uint32_t t0 = 0; np && t0 < nb; t0 += kSmjCap) {
         const int tn = int(min<uint32_t>(kSmjCap, nb - t0));
-        const TileSort ts = gather_tile<true, DIRECT>(s_b, s_p, m.mm, m.dir, part, v, 0, t0, tn);
+        const TileSort ts = gather_tile<true, PER_WINDOW>(s_b, s_p, m.mm, m.dir, part, v, 0, t0, tn);
         if constexpr (GROUPED) {
             // prefix sums of A over the sorted build tile: each thread sums
             // kPer consecutive entries, then a block scan of the sums
@@ -1134,11 +1134,63 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
         for (uint32_t u0 = 0; u0 < np; u0 += kSmjCap) {
             const int un = int(min<uint32_t>(kSmjCap, np - u0));
             gather_tile<false>(s_p, nullptr, nullptr, nullptr, part, v, 1, u0, un);
-            if (!WRITE) {
+            if (!WRITE && !GROUPED && EXPAND) {
+                // a wave in which some probe has more than two matches
+                // expands its match lists cooperatively — lane g of each
+                // round takes match g of the wave's concatenated lists — so
+                // a hot key's list runs on full waves instead of one lane
+                const uint32_t lane = threadIdx.x & (kWave - 1);
+                for (int j0 = 0; j0 < int(un); j0 += kJoinThreads) {
+                    const int j = j0 + int(threadIdx.x);
+                    uint32_t k = 0, sv = 0;
+                    int lo = 0, hi = 0;
+                    if (j < int(un)) {
+                        k = uint32_t(s_p[j] >> 32);
+                        sv = uint32_t(s_p[j]);
+                        lo = lower_bound_key(s_b, tn, k);
+                        hi = upper_bound_key(s_b, lo, tn, k);
+                    }
+                    const uint32_t cnt = uint32_t(hi - lo);
+                    if (__ballot(cnt > 2) == 0) {  // wave-uniform
+                        for (int e = lo; e < hi; e++) {
+                            const uint64_t a = hash_a(k, uint32_t(s_b[e]));
+                            acc.count++;
+                            acc.c1 += a * hash_b(k, sv);
+                            if (C2) acc.c2 += pair_hash_from_a(a, sv);
+                        }
+                        continue;
+                    }
+                    uint32_t tot;
+                    const uint32_t ex = wave_excl_scan(cnt, &tot);
+                    for (uint32_t g0 = 0; g0 < tot; g0 += kWave) {
+                        const uint32_t g = g0 + lane;
+                        // owner: the last lane whose list starts at or before g
+                        int olo = 0, ohi = kWave;
+#pragma unroll
+                        for (int st = 0; st < 6; st++) {
+                            const int mid = (olo + ohi) >> 1;
+                            if (uint32_t(__shfl(int(ex), mid)) <= g)
+                                olo = mid;
+                            else
+                                ohi = mid;
+                        }
+                        const uint32_t e = uint32_t(__shfl(lo, olo)) + (g - uint32_t(__shfl(int(ex), olo)));
+                        const uint32_t svo = uint32_t(__shfl(int(sv), olo));
+                        if (g < tot) {
+                            const uint64_t b = s_b[e];
+                            const uint32_t ko = uint32_t(b >> 32);  // equals the probe's key
+                            const uint64_t a = hash_a(ko, uint32_t(b));
+                            acc.count++;
+                            acc.c1 += a * hash_b(ko, svo);
+                            if (C2) acc.c2 += pair_hash_from_a(a, svo);
+                        }
+                    }
+                }
+            } else if (!WRITE) {
                 for (int j = threadIdx.x; j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
                     int lo, hi;
-                    if (DIRECT && ts.direct) {  // the counting sort's key directory
+                    if (PER_WINDOW && ts.direct) {  // the counting sort's key directory
                         const uint32_t d = k - ts.kmin;
                         if (d >= kSmjDirect) continue;
                         lo = m.dir[d];
@@ -1210,7 +1262,7 @@ struct JoinLdsBytes {
 // spans use the counting sort (the persistent batched join keeps only the
 // sorting networks, whose registers it can hold at three workgroups per CU);
 // the batched join's HJ probe queues every match (its windows are skewed).
-template <int ALG, bool C2, bool WRITE, bool GROUPED, bool PER_WINDOW>
+template <int ALG, bool C2, bool WRITE, bool GROUPED, bool PER_WINDOW, bool EXPAND = false>
 __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, int bshift, uint8_t *mem,
                                           uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
     if constexpr (ALG == SJB_ALG_HJ) {
@@ -1223,7 +1275,8 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            smj_join_part<C2, WRITE, GROUPED, PER_WINDOW>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem), s_ws, s_base,
+            smj_join_part<C2, WRITE, GROUPED, PER_WINDOW, EXPAND>(io, v, *reinterpret_cast<SmjLds<GROUPED> *>(mem),
+                                                                  s_ws, s_base,
                                                       acc);
     }
 }
@@ -1298,7 +1351,7 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
 // the same bits).
 // HJ bodies fit three workgroups per CU in LDS; the register budget is
 // capped to match (6 waves per SIMD -> at most 80 VGPRs).
-template <int ALG, bool C2, bool GROUPED>
+template <int ALG, bool C2, bool GROUPED, bool EXPAND = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kJoinThreads),
                           amdgpu_waves_per_eu(ALG == SJB_ALG_HJ ? SJB_BATCH_HJ_WAVES
                                               : GROUPED        ? 4
@@ -1343,7 +1396,7 @@ k_join_batch(BatchArgs a) {
                 Acc acc;
                 acc.zero();
                 const int bshift = ALG == SJB_ALG_HJ ? logF : 0;
-                join_part<ALG, C2, false, GROUPED, false>(io, v, bshift, s_mem, s_ws, &s_base, acc);
+                join_part<ALG, C2, false, GROUPED, false, EXPAND>(io, v, bshift, s_mem, s_ws, &s_base, acc);
                 const Acc r = block_sum(acc, s_red);
                 if (threadIdx.x == 0 && (r.count | r.c1 | r.c2)) {
                     sjb_window_result_t *o = a.res + bw.id;
@@ -1478,6 +1531,8 @@ int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a
     } else if (alg == SJB_ALG_SMJ) {
         if (grouped)
             SJB_JB(SJB_ALG_SMJ, false, true);
+        else if (c2 && a.expand)
+            k_join_batch<SJB_ALG_SMJ, true, false, true><<<blocks, kJoinThreads, dyn, s>>>(a);
         else if (c2)
             SJB_JB(SJB_ALG_SMJ, true, false);
         else
