@@ -15,6 +15,19 @@ Prints ONE JSON line (rank 0):
   window_ms    p50 / p99 device time per window
   cpu_baseline the C restatement of the reference CPU hash join (oracle),
                timed on a bounded sample of the same workload
+
+Other workloads (--workload; one JSON line each, same contract):
+  --alg SMJ    the same windows through the sort-merge join
+  batch        4096 lognormal windows per launch, Zipf --theta keys,
+               autotuned persistent batched join (HJ or SMJ)
+  learned      learned vs radix partitioning on drifting GMM + Zipf 1.2
+               keys (batched, grouped, HJ and SMJ)
+  slide        eager sliding window (2^22 per side) served from pinned host
+               batches: throughput and p50/p99 per-event latency
+  nlj          lazy nested-loop join at 2^16 windows (compares/s, VALU
+               roofline)
+  eager        eager per-batch joins (HJ, persistent-kernel HJ, NLJ) on
+               host-pinned batches: throughput and p50/p99 per batch
 """
 import argparse
 import json
