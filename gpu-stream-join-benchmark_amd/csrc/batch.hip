@@ -255,7 +255,7 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     a.qhead = p->d_qhead;
     a.chunk = chunk;
     a.grab = uint32_t(std::max(p->tune.grab, 1));
-    a.expand = uint32_t(p->tune.smj_expand);
+    a.expand = uint32_t(p->tune.expand);
     const bool grouped = (p->flags & SJB_F_GROUPED) != 0;
     a.idcap = uint32_t(join_idcap(p->alg, grouped));
     a.rmi = c->d_rmi;
@@ -426,7 +426,7 @@ int sjb_batch_plan_info(sjb_batch_plan_t *p, sjb_batch_info_t *info) {
     info->tune.chunk_log2 = uint32_t(p->tune.chunk_log2);
     info->tune.blocks_per_cu = uint32_t(p->tune.blocks_per_cu);
     info->tune.grab = uint32_t(p->tune.grab);
-    info->tune.smj_expand = uint32_t(p->tune.smj_expand);
+    info->tune.expand = uint32_t(p->tune.expand);
     info->tune.ms = 0.0;
     return SJB_OK;
 }
@@ -445,7 +445,7 @@ int sjb_batch_get_tune(sjb_ctx_t *c, sjb_batch_tune_t *t) {
     t->chunk_log2 = uint32_t(c->tune.chunk_log2);
     t->blocks_per_cu = uint32_t(c->tune.blocks_per_cu);
     t->grab = uint32_t(c->tune.grab);
-    t->smj_expand = uint32_t(c->tune.smj_expand);
+    t->expand = uint32_t(c->tune.expand);
     t->reserved = 0;
     t->ms = 0.0;
     return SJB_OK;
@@ -456,12 +456,12 @@ int sjb_batch_set_tune(sjb_ctx_t *c, const sjb_batch_tune_t *t) {
     if (t->fanout_bias < -4 || t->fanout_bias > 4) return SJB_E_INVALID;
     if (t->chunk_log2 != 12 && t->chunk_log2 != 14) return SJB_E_INVALID;
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 3 || t->grab < 1 || t->grab > 64) return SJB_E_INVALID;
-    if (t->smj_expand > 1) return SJB_E_INVALID;
+    if (t->expand > 1) return SJB_E_INVALID;
     c->tune.fanout_bias = t->fanout_bias;
     c->tune.chunk_log2 = int(t->chunk_log2);
     c->tune.blocks_per_cu = int(t->blocks_per_cu);
     c->tune.grab = int(t->grab);
-    c->tune.smj_expand = int(t->smj_expand);
+    c->tune.expand = int(t->expand);
     return SJB_OK;
 }
 
@@ -469,8 +469,8 @@ int sjb_batch_set_tune(sjb_ctx_t *c, const sjb_batch_tune_t *t) {
 // run, then `reps` timed runs, median), checks that every candidate gives
 // bit-identical per-window results, keeps the fastest in the ctx and
 // reports it.  Candidates: fanout bias {-1,0,+1} x chunk {2^12,2^14} x
-// join workgroups per CU {1,2,3} x queue grab {1,4} (x cooperative match
-// expansion {off, on} for SMJ with C2, not grouped).
+// join workgroups per CU {1,2,3} x queue grab {1,4} (x long match lists on
+// full waves {off, on} for non-grouped HJ and SMJ with C2).
 int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t *d_left,
                        const uint64_t *left_off, const sjb_tuple_t *d_right,
                        const uint64_t *right_off, uint32_t n_windows, uint32_t reps,
@@ -484,11 +484,11 @@ int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t 
     BatchTune best_t = c->tune;
     double best_ms = 1e300;
     uint32_t tried = 0;
-    // SMJ with C2 and no grouping also tries the cooperative expansion of
-    // long match lists (it pays under skewed keys, costs a little without)
+    // non-grouped HJ and SMJ with C2 also try running long match lists on
+    // full waves (it pays under skewed keys, costs a little without)
     const uint32_t fl = flags | c->opts.flags;
     const int n_expand =
-        alg == SJB_ALG_SMJ && (fl & SJB_F_C2) && !(fl & SJB_F_GROUPED) ? 2 : 1;
+        !(fl & SJB_F_GROUPED) && (alg == SJB_ALG_HJ || (fl & SJB_F_C2)) ? 2 : 1;
     for (int bias = -1; bias <= 1; bias++)
         for (int cl : {12, 14})
             for (int bpc : {1, 2, 3})
@@ -499,7 +499,7 @@ int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t 
                     t.chunk_log2 = cl;
                     t.blocks_per_cu = bpc;
                     t.grab = grab;
-                    t.smj_expand = ex;
+                    t.expand = ex;
                     rc = batch_run(c, alg, flags, d_left, left_off, d_right, right_off, n_windows, t);
                     if (rc == SJB_E_CAPACITY) continue;  // window too large for this chunk
                     if (rc) return rc;
@@ -534,7 +534,7 @@ int sjb_batch_autotune(sjb_ctx_t *c, int alg, uint32_t flags, const sjb_tuple_t 
     best->chunk_log2 = uint32_t(best_t.chunk_log2);
     best->blocks_per_cu = uint32_t(best_t.blocks_per_cu);
     best->grab = uint32_t(best_t.grab);
-    best->smj_expand = uint32_t(best_t.smj_expand);
+    best->expand = uint32_t(best_t.expand);
     best->reserved = 0;
     best->ms = best_ms;
     if (n_tried) *n_tried = tried;

@@ -370,9 +370,10 @@ __device__ __forceinline__ void pairq_drain(const PairQ &q, uint32_t n, Acc &acc
 // thread (element j = k * blockDim + tid < un) against the current tile
 // (payload = val).  The bucket scan compares keys only; the first two
 // matches of every probe go to the wave's queue (later matches, rare under
-// uniform keys, are summed in place).  ALLQ (the batched join, built for
-// skewed windows): every match goes through the queue instead — fewer
-// idle lanes under hot keys, ~5% more work under uniform keys.
+// uniform keys, are summed in place).  ALLQ (a batched-join variant the
+// autotuner picks for skewed windows): every match goes through the queue
+// instead — fewer idle lanes under hot keys, ~5% more work under uniform
+// keys.
 template <bool C2, bool ALLQ>
 __device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[kHjPerP], uint32_t un,
                                               int bshift, Acc &acc, PairQ &q) {
@@ -1261,7 +1262,8 @@ struct JoinLdsBytes {
 // PER_WINDOW: the per-window join (k_join) — SMJ build tiles with narrow key
 // spans use the counting sort (the persistent batched join keeps only the
 // sorting networks, whose registers it can hold at three workgroups per CU);
-// the batched join's HJ probe queues every match (its windows are skewed).
+// EXPAND (batched join only) runs long match lists on full waves: the HJ
+// probe queues every match, the SMJ probe expands them cooperatively.
 template <int ALG, bool C2, bool WRITE, bool GROUPED, bool PER_WINDOW, bool EXPAND = false>
 __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, int bshift, uint8_t *mem,
                                           uint32_t *s_ws, unsigned long long *s_base, Acc &acc) {
@@ -1269,7 +1271,7 @@ __device__ __forceinline__ void join_part(const JoinIO &io, const PartView &v, i
         if constexpr (GROUPED)
             agg_join_part(io, v, bshift, *reinterpret_cast<AggLds *>(mem), acc);
         else
-            hj_join_part<C2, WRITE, !PER_WINDOW>(io, v, bshift, *reinterpret_cast<HjLds<WRITE> *>(mem), s_ws,
+            hj_join_part<C2, WRITE, EXPAND>(io, v, bshift, *reinterpret_cast<HjLds<WRITE> *>(mem), s_ws,
                                                  s_base, acc);
     } else {
         if (GROUPED && v.n[0] > uint32_t(kSmjCap))
@@ -1524,6 +1526,9 @@ int launch_join_batch(hipStream_t s, int alg, uint32_t flags, const BatchArgs &a
     if (alg == SJB_ALG_HJ) {
         if (grouped)
             SJB_JB(SJB_ALG_HJ, false, true);
+        else if (a.expand)
+            c2 ? k_join_batch<SJB_ALG_HJ, true, false, true><<<blocks, kJoinThreads, dyn, s>>>(a)
+               : k_join_batch<SJB_ALG_HJ, false, false, true><<<blocks, kJoinThreads, dyn, s>>>(a);
         else if (c2)
             SJB_JB(SJB_ALG_HJ, true, false);
         else

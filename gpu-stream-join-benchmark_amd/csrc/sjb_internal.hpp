@@ -133,7 +133,7 @@ struct BatchArgs {
     uint32_t qbeg[kXcds + 1];    // join items of XCD x: [qbeg[x], qbeg[x+1])
     uint32_t chunk;              // partition-pass chunk (tuples)
     uint32_t grab;               // join items taken per queue access
-    uint32_t expand;             // SMJ, C2, not grouped: cooperative match expansion
+    uint32_t expand;             // non-grouped: long match lists on full waves
     uint32_t idcap;              // run-id table entries of a PartView
     const RmiParams *rmi;
     uint64_t *heavy;             // learned + grouped: 4 u64 per (window, heavy key)
@@ -145,7 +145,7 @@ struct BatchTune {
     int chunk_log2 = 14;       // partition-pass chunk: 2^12 or 2^14 tuples
     int blocks_per_cu = 2;     // resident join workgroups per CU
     int grab = 1;              // join items per queue access
-    int smj_expand = 0;        // SMJ (C2, not grouped): cooperative match expansion
+    int expand = 0;            // non-grouped: long match lists on full waves
 };
 
 struct WindowPlan {

@@ -57,7 +57,7 @@ class Options(ctypes.Structure):
 class BatchTune(ctypes.Structure):
     _fields_ = [("fanout_bias", ctypes.c_int32), ("chunk_log2", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("grab", ctypes.c_uint32),
-                ("smj_expand", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double)]
+                ("expand", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

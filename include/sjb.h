@@ -219,9 +219,10 @@ typedef struct sjb_batch_tune {
     uint32_t chunk_log2;    /* partition-pass chunk: 12 or 14                */
     uint32_t blocks_per_cu; /* resident persistent join workgroups per CU    */
     uint32_t grab;          /* join work items taken per queue access        */
-    uint32_t smj_expand;    /* SMJ with C2, not grouped: 1 = a wave whose    */
-                            /* probes have long match lists expands them     */
-                            /* cooperatively (skewed keys); 0 or 1           */
+    uint32_t expand;        /* non-grouped joins, 0 or 1: 1 = long match     */
+                            /* lists run on full waves (HJ: every match via  */
+                            /* the wave's queue; SMJ with C2: cooperative    */
+                            /* expansion) — pays under skewed keys           */
     uint32_t reserved;      /* 0                                             */
     double ms;              /* autotune: median batch time of this shape     */
 } sjb_batch_tune_t;

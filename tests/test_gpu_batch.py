@@ -128,7 +128,8 @@ def test_autotune_agrees(ctx, torch_mod, sjbmod):
     base = ctx.lazy_join_batch(HJ, F_C2, dR, lo, dS, ro)
     best, tried = ctx.batch_autotune(HJ, F_C2, dR, lo, dS, ro, reps=2)
     # fanout bias {-1,0,1} x chunk {2^12,2^14} x workgroups/CU {1,2,3} x grab {1,4}
-    assert tried == 36 and best.ms > 0
+    # x match lists on full waves {off, on}
+    assert tried == 72 and best.ms > 0
     t = ctx.batch_tune()
     assert (t.fanout_bias, t.chunk_log2, t.blocks_per_cu, t.grab) == \
         (best.fanout_bias, best.chunk_log2, best.blocks_per_cu, best.grab)
@@ -142,27 +143,31 @@ def test_tune_validation(ctx):
     with pytest.raises(sjb_error()):
         ctx.batch_tune(blocks_per_cu=4)
     with pytest.raises(sjb_error()):
-        ctx.batch_tune(smj_expand=2)
+        ctx.batch_tune(expand=2)
 
 
-def test_smj_expansion_matches(ctx, torch_mod, orc, sjbmod):
-    """Batched SMJ with C2 on skewed keys: the cooperative expansion of long
-    match lists gives the same per-window results as the per-lane loop, and
-    the autotuner tries both for SMJ (72 shapes)."""
+@pytest.mark.parametrize("alg,flags", [(HJ, 0), (HJ, F_C2), (SMJ, F_C2)])
+def test_expansion_matches(ctx, torch_mod, orc, sjbmod, alg, flags):
+    """Batched joins on skewed keys: running long match lists on full waves
+    (HJ: every match through the wave's queue; SMJ: cooperative expansion)
+    gives the same per-window results as the per-lane loop, and the
+    autotuner tries both (72 shapes)."""
     R, S, lo, ro = batch_case(sjbmod, 96, 0.8, seed=23)
     dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
     res = {}
     for ex in (0, 1):
-        ctx.batch_tune(fanout_bias=0, chunk_log2=12, blocks_per_cu=3, grab=1, smj_expand=ex)
-        res[ex] = ctx.lazy_join_batch(SMJ, F_C2, dR, lo, dS, ro)
+        ctx.batch_tune(fanout_bias=0, chunk_log2=12, blocks_per_cu=3, grab=1, expand=ex)
+        res[ex] = ctx.lazy_join_batch(alg, flags, dR, lo, dS, ro)
     assert np.array_equal(res[0], res[1])
     for w in (0, 17, 95):
         o, _ = orc.lazy_hj(R[lo[w]:lo[w + 1]], S[ro[w]:ro[w + 1]], 1000)
-        assert (int(res[1][w]["count"]), int(res[1][w]["checksum1"]), int(res[1][w]["checksum2"])) == o.triple()
-    best, tried = ctx.batch_autotune(SMJ, F_C2, dR, lo, dS, ro, reps=1)
-    assert tried == 72 and best.smj_expand in (0, 1)
-    assert np.array_equal(ctx.lazy_join_batch(SMJ, F_C2, dR, lo, dS, ro), res[0])
-    ctx.batch_tune(fanout_bias=0, chunk_log2=14, blocks_per_cu=2, grab=1, smj_expand=0)
+        want = o.triple() if flags & F_C2 else o.triple()[:2]
+        got = (int(res[1][w]["count"]), int(res[1][w]["checksum1"]), int(res[1][w]["checksum2"]))
+        assert got[:len(want)] == want
+    best, tried = ctx.batch_autotune(alg, flags, dR, lo, dS, ro, reps=1)
+    assert tried == 72 and best.expand in (0, 1)
+    assert np.array_equal(ctx.lazy_join_batch(alg, flags, dR, lo, dS, ro), res[0])
+    ctx.batch_tune(fanout_bias=0, chunk_log2=14, blocks_per_cu=2, grab=1, expand=0)
 
 
 def sjb_error():
