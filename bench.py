@@ -600,8 +600,10 @@ def main():
     # Algorithmic bytes per launch: partition kernel reads both sides' tuples
     # (16 B) and writes one 8-byte record per tuple; the join phase (per-
     # partition join + reduction) reads every record once.
-    phases = {"partition": {"alg_bytes": 24 * 2 * W, "ms": float(part_ms.mean())},
-              "join": {"alg_bytes": 8 * 2 * W, "ms": float(join_ms.mean())}}
+    phases = {"partition": {"alg_bytes": 24 * 2 * W, "ms": float(part_ms.mean()),
+                            "kernels": "partition pass (k_partition_pipe; SMJ: + key-range sample)"},
+              "join": {"alg_bytes": 8 * 2 * W, "ms": float(join_ms.mean()),
+                       "kernels": "k_join + k_reduce (one event pair around both)"}}
     dom = max(phases, key=lambda k: phases[k]["ms"])
     for v in phases.values():
         v["achieved_GBps"] = v["alg_bytes"] / (v["ms"] / 1e3) / 1e9
@@ -636,7 +638,7 @@ def main():
         "window_ms": {"p50": float(np.percentile(win_ms, 50)), "p99": float(np.percentile(win_ms, 99)),
                       "mean": float(win_ms.mean()), "timed_windows": int(len(win_ms)),
                       "sampling": f"1 in {TIME_EVERY} windows carries timestamp events"},
-        "roofline": {"bound": "hbm", "kernel": dom,
+        "roofline": {"bound": "hbm", "kernel": dom, "kernels_timed": phases[dom]["kernels"],
                      "achieved": phases[dom]["achieved_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": phases[dom]["frac"], "traffic": traffic,
                      "alg_bytes_per_launch": phases[dom]["alg_bytes"],
