@@ -117,7 +117,9 @@ def learned_case(a, first, nw, torch, sjb, np):
             rep = ctx.partition_report(alg, f, dR, W, dS, W)
             best, tried = ctx.batch_autotune(alg, f, dR, off, dS, off, reps=2)
             d_res = torch.zeros(nw * 32, dtype=torch.uint8, device="cuda")
+            t_plan = time.perf_counter()
             plan = ctx.batch_plan(alg, f, dR, off, dS, off, d_res)
+            plan_ms = 1e3 * (time.perf_counter() - t_plan)
             pinfo = plan.info()
             ms = time_plan(ctx, plan, a.steps, a.warmup, torch)
             r = d_res.cpu().numpy().view(sjb.WRES_DTYPE).copy()
@@ -130,6 +132,8 @@ def learned_case(a, first, nw, torch, sjb, np):
                          "tune": best.as_dict(),
                          "plan_partitioning": ["hash", "radix ranges", "learned CDF"][pinfo.part_kind],
                          "drift_fallback": bool(pinfo.drift_fallback),
+                         "drift_refit": bool(pinfo.drift_refit),
+                         "plan_build_ms": plan_ms,
                          "results": [(int(x["count"]), int(x["checksum1"])) for x in r]}
         if out["radix"]["results"] != out["learned"]["results"]:
             raise SystemExit(f"{name}: learned and radix results differ")
@@ -148,7 +152,9 @@ def bench_learned(a):
     leaves, up to 32 heavy keys) is fitted on the 4 windows before the timed
     batch of 8 windows.  Headline case: windows 68..75 (same drift epoch as the
     training windows 64..67); also reported: windows 64..71 right after a drift
-    step, trained on 60..63.  Count + C1 (grouped).  A step is one replay of the
+    step, trained on 60..63 (the plan finds the model stale on its own key
+    sample and refits it to that sample: drift_refit; plan_build_ms includes
+    the sample, the judgement and the refit).  Count + C1 (grouped).  A step is one replay of the
     batch plan over the 8 windows."""
     import numpy as np
     import torch

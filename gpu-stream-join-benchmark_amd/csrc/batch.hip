@@ -40,7 +40,11 @@ struct sjb_batch_plan {
     uint64_t *d_heavy = nullptr;      // learned + grouped: per-window heavy-key sums
     size_t cap_heavy = 0;
     bool peel = false;
-    bool drift_fallback = false;      // learned CDF judged stale on this batch
+    bool drift_fallback = false;      // learned CDF judged stale on this batch, radix ranges used
+    bool drift_refit = false;         // ... and refitted to this batch's key sample instead
+    sjb::RmiParams rmi_refit{};       // the refitted model (host copy)
+    sjb::RmiParams *d_rmi = nullptr;  // ... and on the device
+    size_t cap_rmi = 0;
     uint32_t n_heavy = 0;
     size_t cap_win = 0, cap_chunks = 0, cap_items = 0, cap_part = 0, cap_offs = 0;
     uint64_t dev_bytes = 0;
@@ -86,7 +90,8 @@ void batch_plan_free(sjb_batch_plan *p) {
     if (p->c) p->c->ext_bytes -= p->dev_bytes;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *dev[] = {p->d_win, p->d_chunk_win, p->d_item_win, p->d_part, p->d_offs, p->d_qhead, p->d_heavy};
+    void *dev[] = {p->d_win, p->d_chunk_win, p->d_item_win, p->d_part, p->d_offs, p->d_qhead, p->d_heavy,
+                   p->d_rmi};
     for (void *d : dev)
         if (d) (void)hipFree(d);
     if (p->h_stage) (void)hipHostFree(p->h_stage);
@@ -109,18 +114,24 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     }
     const uint64_t nl_tot = loff[n] - loff[0], nr_tot = roff[n] - roff[0];
     if (nl_tot + nr_tot >= (1ull << 32)) return SJB_E_CAPACITY;
-    // The learned CDF judges itself on this batch's keys: a strided sample
-    // of the left stream; if it partitions worse than radix ranges (stale
-    // after a drift), the batch uses radix ranges (peeling stays).
+    // The learned CDF judges itself on this batch's keys (an evenly spaced
+    // sample of the left stream).  Stale after a drift, it is refitted to
+    // that sample (root = the sample's CDF, leaves = chords of it; a
+    // plan-private copy), or — with SJB_F_NO_REFIT, or when the refit does
+    // not beat radix — the batch uses radix ranges.  Peeled heavy keys stay.
     p->drift_fallback = false;
+    p->drift_refit = false;
     if (p->kind == PART_LEARNED && nl_tot > 0) {
-        const uint32_t ns = uint32_t(std::min<uint64_t>(nl_tot, 8192));
-        const uint64_t stride = nl_tot / ns;
-        std::vector<uint32_t> keys(ns);
-        SJB_HIP_TRY(hipMemcpy2DAsync(keys.data(), sizeof(uint32_t), L + loff[0], stride * sizeof(sjb_tuple_t),
-                                     sizeof(uint32_t), ns, hipMemcpyDeviceToHost, c->stream));
-        SJB_HIP_TRY(hipStreamSynchronize(c->stream));
-        if (rmi_drifted(c, keys, p->peel)) {
+        std::vector<uint32_t> keys;
+        int rc = rmi_sample_keys(c, L + loff[0], nl_tot, keys);
+        if (rc) return rc;
+        const int verdict = rmi_judge(c, keys, p->peel, !(p->flags & SJB_F_NO_REFIT), &p->rmi_refit);
+        if (verdict == kDriftRefit) {
+            if ((rc = grow(p, &p->d_rmi, &p->cap_rmi, 1))) return rc;
+            SJB_HIP_TRY(hipMemcpyAsync(p->d_rmi, &p->rmi_refit, sizeof(RmiParams), hipMemcpyHostToDevice,
+                                       c->stream));
+            p->drift_refit = true;
+        } else if (verdict == kDriftRanges) {
             p->kind = PART_RANGE;
             p->drift_fallback = true;
         }
@@ -258,7 +269,7 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     a.expand = uint32_t(p->tune.expand);
     const bool grouped = (p->flags & SJB_F_GROUPED) != 0;
     a.idcap = uint32_t(join_idcap(p->alg, grouped));
-    a.rmi = c->d_rmi;
+    a.rmi = p->drift_refit ? p->d_rmi : c->d_rmi;
     a.heavy = p->peel ? p->d_heavy : nullptr;
     p->dyn_part = partition_lds_bytes_chunk(max_logF, int(chunk));
     p->dyn_join = join_view_bytes(p->alg, grouped, max_ncL, max_ncR);
@@ -422,6 +433,8 @@ int sjb_batch_plan_info(sjb_batch_plan_t *p, sjb_batch_info_t *info) {
     info->device_bytes = p->dev_bytes;
     info->part_kind = uint32_t(p->kind);
     info->drift_fallback = p->drift_fallback ? 1u : 0u;
+    info->drift_refit = p->drift_refit ? 1u : 0u;
+    info->reserved = 0;
     info->tune.fanout_bias = p->tune.fanout_bias;
     info->tune.chunk_log2 = uint32_t(p->tune.chunk_log2);
     info->tune.blocks_per_cu = uint32_t(p->tune.blocks_per_cu);

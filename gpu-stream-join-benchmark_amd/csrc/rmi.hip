@@ -382,14 +382,8 @@ static void heavy_table_build(RmiParams &m) {
 }
 
 
-// Drift check of the learned CDF on the keys a batch will partition: the
-// partition imbalance (max / mean over Fv parts) of a strided key sample
-// under the model against MSD radix on the sample's key range; heavy keys
-// are left out when they are peeled.  Returns true when the model is more
-// than 1.25x as imbalanced as radix (the batch then uses radix ranges).
-bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel) {
-    if (c->rmi_host.size() != sizeof(RmiParams) || keys.size() < 256) return false;
-    const RmiParams &m = *reinterpret_cast<const RmiParams *>(c->rmi_host.data());
+// Sample keys that the model partitions (heavy keys left out when peeled).
+static std::vector<uint32_t> model_keys(const RmiParams &m, const std::vector<uint32_t> &keys, bool peel) {
     std::vector<uint32_t> rest;
     rest.reserve(keys.size());
     for (uint32_t k : keys) {
@@ -397,7 +391,13 @@ bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel)
         for (uint32_t h = 0; peel && h < m.n_heavy; h++) heavy |= m.heavy[h] == k;
         if (!heavy) rest.push_back(k);
     }
-    if (rest.size() < 256) return false;
+    return rest;
+}
+
+// Partition imbalance (max / mean over Fv parts) of `rest` under model m
+// and under MSD radix on the keys' range.
+static void sample_imbalance(const RmiParams &m, const std::vector<uint32_t> &rest, double *learned,
+                             double *radix) {
     const uint32_t Fv = 256;
     uint32_t kmin = rest[0], kmax = rest[0];
     for (uint32_t k : rest) {
@@ -412,7 +412,101 @@ bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel)
         hl[host_partition(m, k, Fv)]++;
         hr[std::min<uint32_t>((k - kmin) >> shift, Fv - 1)]++;
     }
-    return imbalance(hl) > 1.25 * imbalance(hr);
+    *learned = imbalance(hl);
+    *radix = imbalance(hr);
+}
+
+// Evenly spaced sample of up to kFitSample keys of T[0, n), copied to the
+// host (uses the fit's key buffer; needs a fitted model).
+int rmi_sample_keys(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, std::vector<uint32_t> &keys) {
+    if (!c->d_fit_keys || n == 0) return SJB_E_INVALID;
+    const uint32_t ns = uint32_t(std::min<uint64_t>(n, kFitSample));
+    k_rmi_sample<<<(ns + 255) / 256, 256, 0, c->stream>>>(T, n, ns, c->d_fit_keys);
+    SJB_HIP_TRY(hipGetLastError());
+    keys.resize(ns);
+    SJB_HIP_TRY(hipMemcpyAsync(keys.data(), c->d_fit_keys, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
+    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SJB_OK;
+}
+
+// Refit of model m to the sorted keys `rest`, without training: the root
+// becomes their empirical CDF over x (fraction of keys below x = i /
+// kRootTable, interpolated linearly), and leaf l the chord of that root
+// between the x where it reaches l / kRmiLeaves and the x where it reaches
+// (l + 1) / kRmiLeaves, so the two stages follow the keys' quantiles to
+// within 1 / kRmiLeaves.  Heavy keys and their tables are left as they are.
+static void refit_to(RmiParams &m, const std::vector<uint32_t> &rest) {
+    const size_t ns = rest.size();
+    m.kmin = rest.front();
+    m.scale = float(1.0 / (double(rest.back() - rest.front()) + 1.0));
+    m.root_kind = ROOT_TABLE;
+    std::vector<float> xs(ns);
+    for (size_t j = 0; j < ns; j++) xs[j] = float(rest[j] - m.kmin) * m.scale;
+    for (int i = 0; i <= kRootTable; i++) {
+        const float xi = float(i) / float(kRootTable);
+        const size_t below = size_t(std::lower_bound(xs.begin(), xs.end(), xi) - xs.begin());
+        m.table[i] = float(double(below) / double(ns));
+    }
+    // x at which the (non-decreasing, piecewise linear) root reaches v;
+    // called with non-decreasing v, so the segment search resumes
+    int i = 0;
+    auto root_inv = [&](double v) {
+        if (v <= double(m.table[0])) return 0.0;
+        while (i < kRootTable - 1 && double(m.table[i + 1]) < v) i++;
+        const double lo = m.table[i], hi = m.table[i + 1];
+        const double f = hi > lo ? std::min(1.0, std::max(0.0, (v - lo) / (hi - lo))) : 0.0;
+        return (double(i) + f) / double(kRootTable);
+    };
+    for (int l = 0; l < kRmiLeaves; l++) {
+        const double v0 = double(l) / kRmiLeaves, v1 = double(l + 1) / kRmiLeaves;
+        const double x0 = root_inv(v0), x1 = root_inv(v1);
+        if (x1 - x0 > 1e-12) {
+            const double a = (v1 - v0) / (x1 - x0);
+            m.leaf_a[l] = float(a);
+            m.leaf_b[l] = float(v0 - a * x0);
+        } else {
+            m.leaf_a[l] = 0.f;
+            m.leaf_b[l] = float((l + 0.5) / kRmiLeaves);
+        }
+    }
+}
+
+// Drift judgement of the learned CDF on the keys a batch will partition (a
+// key sample; heavy keys left out when they are peeled).  The sample's odd
+// half is scored (partition imbalance, max / mean over 256 parts) under the
+// model, under MSD radix on its key range and, when refits are allowed,
+// under a refit of the model to the even half.  Returns
+//   kDriftRefit  the model is more than 1.5x as imbalanced as the refit,
+//                and the refit within 1.25x of radix: *out = the model
+//                refitted to the whole sample;
+//   kDriftRanges the model (no refit allowed or the refit is no good) is
+//                more than 1.25x as imbalanced as radix;
+//   kDriftNone   otherwise (the model is used as it is).
+int rmi_judge(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel, bool allow_refit,
+              RmiParams *out) {
+    if (c->rmi_host.size() != sizeof(RmiParams) || keys.size() < 512) return kDriftNone;
+    const RmiParams &m = *reinterpret_cast<const RmiParams *>(c->rmi_host.data());
+    std::vector<uint32_t> rest = model_keys(m, keys, peel);
+    if (rest.size() < 512) return kDriftNone;
+    std::vector<uint32_t> even, odd;
+    even.reserve(rest.size() / 2 + 1);
+    odd.reserve(rest.size() / 2);
+    for (size_t j = 0; j < rest.size(); j++) (j & 1 ? odd : even).push_back(rest[j]);
+    double il, ir;
+    sample_imbalance(m, odd, &il, &ir);
+    if (allow_refit) {
+        std::memcpy(out, &m, sizeof(RmiParams));
+        std::sort(even.begin(), even.end());
+        refit_to(*out, even);
+        double ifit, unused;
+        sample_imbalance(*out, odd, &ifit, &unused);
+        if (il > 1.5 * ifit && ifit <= 1.25 * ir) {
+            std::sort(rest.begin(), rest.end());
+            refit_to(*out, rest);
+            return kDriftRefit;
+        }
+    }
+    return il > 1.25 * ir ? kDriftRanges : kDriftNone;
 }
 }  // namespace sjb
 

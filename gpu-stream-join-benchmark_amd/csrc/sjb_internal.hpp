@@ -320,7 +320,11 @@ int launch_batch_clear(hipStream_t s, sjb_window_result_t *res, uint32_t n, uint
                        uint64_t *heavy);
 size_t partition_lds_bytes_chunk(int logF, int chunk);
 size_t join_view_bytes(int alg, bool grouped, int ncL, int ncR);
-bool rmi_drifted(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel);
+// verdicts of rmi_judge
+constexpr int kDriftNone = 0, kDriftRefit = 1, kDriftRanges = 2;
+int rmi_judge(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel, bool allow_refit,
+              RmiParams *out);
+int rmi_sample_keys(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, std::vector<uint32_t> &keys);
 int join_idcap(int alg, bool grouped);
 
 // ---- pointer checks / output staging (api.hip) ----

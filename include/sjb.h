@@ -74,6 +74,9 @@ extern "C" {
                                /* pass may run on a second stream, overlapping */
                                /* the previous window's join; the inputs must  */
                                /* be ready when the call is made               */
+#define SJB_F_NO_REFIT 0x20u   /* batched + SJB_F_LEARNED: a model found stale */
+                               /* on the batch's keys is not refitted to them; */
+                               /* the batch uses radix ranges instead          */
 
 /* ---- data layout ------------------------------------------------------- */
 /* Identical to Tuple (include/sjb/utils/Tuple.hpp:6-11): 16 B, AoS.        */
@@ -237,8 +240,15 @@ typedef struct sjb_batch_info {
     uint32_t part_kind;     /* partitioning used: 0 hash, 1 radix ranges,    */
                             /* 2 learned CDF                                 */
     uint32_t drift_fallback;/* 1: SJB_F_LEARNED asked, but the model was     */
-                            /* more imbalanced than radix ranges on a sample */
-                            /* of this batch's keys, so ranges are used      */
+                            /* stale on a sample of this batch's keys and no */
+                            /* refit was made or helped: more imbalanced     */
+                            /* than radix ranges, so ranges are used         */
+    uint32_t drift_refit;   /* 1: the model was stale on that sample, so the */
+                            /* plan refitted a private copy of it to the     */
+                            /* sample (root = the sample's CDF as a table,   */
+                            /* leaves = chords of it; heavy keys kept) and   */
+                            /* partitions by it                              */
+    uint32_t reserved;
 } sjb_batch_info_t;
 
 int sjb_batch_plan_create(sjb_ctx_t *ctx, int algorithm, uint32_t flags,
@@ -273,7 +283,11 @@ int sjb_batch_set_tune(sjb_ctx_t *ctx, const sjb_batch_tune_t *tune);
  * heavy_min times) are aggregated apart instead of partitioned.  The fit compares the learned partition of the
  * sample with MSD radix on the key bits; when the learned one is more than
  * 1.25x as imbalanced the model is not used (report->fallback = 1) and
- * SJB_F_LEARNED joins take the plain radix / hash path.  Results never
+ * SJB_F_LEARNED joins take the plain radix / hash path.  Batch plans judge
+ * the model again on a sample of their own keys and, when it has gone stale
+ * (a drift since the fit), partition by a copy refitted to that sample, or
+ * by radix ranges (sjb_batch_info_t.drift_refit / drift_fallback;
+ * SJB_F_NO_REFIT).  Results never
  * depend on the model: only partition balance does.  No reference
  * counterpart (the reference partitions by key % D only).                 */
 #define SJB_RMI_LINEAR 0

@@ -47,6 +47,10 @@ def test_struct_layouts(sjbmod):
     assert sjbmod.RESULT_DTYPE.itemsize == 32
     assert ctypes.sizeof(sjbmod.Stats) == 9 * 8 + 8
     assert ctypes.sizeof(sjbmod.Options) == 4 * 8 + 8
+    # sjb_batch_tune_t: int32 + 5 u32 + double; sjb_batch_info_t: 4 u32,
+    # u64, the tune, 4 u32
+    assert ctypes.sizeof(sjbmod.BatchTune) == 32
+    assert ctypes.sizeof(sjbmod.BatchInfo) == 72
 
 
 @pytest.mark.skipif(_gpu_available(), reason="checks the no-device error path")

@@ -143,30 +143,37 @@ def test_trained_network_gains(sjbmod, torch_mod):
     assert mlp.fixup_frac < lin.fixup_frac < radix.fixup_frac
 
 
-def test_drift_judge_falls_back_to_ranges(fitted, torch_mod, sjbmod):
+def test_drift_judge_refits_or_falls_back(fitted, torch_mod, sjbmod):
     # The model was fitted on windows 0..3; 640 windows later the mixture
     # means have moved by 10% of the key range and the learned CDF is
-    # stale: the batch plan judges it on a sample of its own keys and uses
-    # radix ranges instead (heavy keys stay peeled); results do not change.
+    # stale: the batch plan judges it on a sample of its own keys and
+    # refits a private copy to that sample (learned CDF kept), or with
+    # SJB_F_NO_REFIT uses radix ranges (heavy keys stay peeled either way);
+    # results do not change.
     c, _, _, _ = fitted
     n, nw = 1 << 16, 4
     lo = np.arange(nw + 1, dtype=np.uint64) * np.uint64(n)
+    F_NO_REFIT = sjbmod.F_NO_REFIT
     for first, stale in ((4, False), (640, True)):
         R = np.concatenate([sjbmod.generate_mix(n, w, seed=931 + w) for w in range(first, first + nw)])
         S = np.concatenate([sjbmod.generate_mix(n, w, seed=951 + w) for w in range(first, first + nw)])
         dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
         got = {}
-        for f in (F_GROUPED | F_LEARNED, F_GROUPED):
+        for f in (F_GROUPED | F_LEARNED, F_GROUPED | F_LEARNED | F_NO_REFIT, F_GROUPED):
             d_res = torch_mod.zeros(nw * 32, dtype=torch_mod.uint8, device="cuda")
             plan = c.batch_plan(SMJ, f, dR, lo, dS, lo, d_res)
             plan.launch()
             c.sync()
             info = plan.info()
-            if f & F_LEARNED:
-                assert info.drift_fallback == int(stale), (first, info.part_kind)
-                assert info.part_kind == (1 if stale else 2)
+            if not f & F_LEARNED:
+                assert (info.drift_fallback, info.drift_refit, info.part_kind) == (0, 0, 1)
+            elif not stale:
+                assert (info.drift_fallback, info.drift_refit, info.part_kind) == (0, 0, 2), first
+            elif f & F_NO_REFIT:
+                assert (info.drift_fallback, info.drift_refit, info.part_kind) == (1, 0, 1)
             else:
-                assert info.drift_fallback == 0 and info.part_kind == 1
+                assert (info.drift_fallback, info.drift_refit, info.part_kind) == (0, 1, 2)
             got[f] = d_res.cpu().numpy().view(sjbmod.WRES_DTYPE)[["count", "checksum1"]].copy()
             plan.destroy()
         assert np.array_equal(got[F_GROUPED | F_LEARNED], got[F_GROUPED])
+        assert np.array_equal(got[F_GROUPED | F_LEARNED | F_NO_REFIT], got[F_GROUPED])
