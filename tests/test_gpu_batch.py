@@ -170,6 +170,20 @@ def test_expansion_matches(ctx, torch_mod, orc, sjbmod, alg, flags):
     ctx.batch_tune(fanout_bias=0, chunk_log2=14, blocks_per_cu=2, grab=1, expand=0)
 
 
+@pytest.mark.parametrize("alg,flags", [(HJ, F_GROUPED), (SMJ, 0), (SMJ, F_GROUPED)])
+def test_expand_ignored_where_it_has_no_variant(ctx, torch_mod, sjbmod, alg, flags):
+    """Grouped joins and SMJ without C2 have no full-wave variant: a tune
+    with expand = 1 runs the same kernel and gives the same results."""
+    R, S, lo, ro = batch_case(sjbmod, 64, 0.8, seed=29)
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    res = {}
+    for ex in (0, 1):
+        ctx.batch_tune(fanout_bias=0, chunk_log2=12, blocks_per_cu=3, grab=1, expand=ex)
+        res[ex] = ctx.lazy_join_batch(alg, flags, dR, lo, dS, ro)
+    ctx.batch_tune(fanout_bias=0, chunk_log2=14, blocks_per_cu=2, grab=1, expand=0)
+    assert np.array_equal(res[0], res[1])
+
+
 def sjb_error():
     import sjb
     return sjb.SjbError
