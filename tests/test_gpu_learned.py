@@ -12,6 +12,9 @@ root trained on the device).
   trained MLP root gives lower partition imbalance and fewer fix-ups
   (tuples in partitions larger than one LDS tile) than the linear-root RMI
   and than radix (range) partitioning on the key bits.
+* Drift: a batch plan judges the model on a sample of its own keys and
+  refits a stale one to that sample, or falls back to radix ranges with
+  SJB_F_NO_REFIT; neither changes the results.
 """
 import numpy as np
 import pytest
