@@ -41,6 +41,28 @@ int dev_alloc(sjb_ctx *c, void **p, size_t bytes) {
     return SJB_OK;
 }
 
+// Frees a tracked device buffer of `bytes` bytes (as passed to dev_alloc)
+// and gives its bytes back to the ctx's account; *p becomes null.
+void dev_free(sjb_ctx *c, void **p, size_t bytes) {
+    if (!*p) return;
+    if (bytes == 0) bytes = 256;
+    auto &v = c->dev_allocs;
+    v.erase(std::remove(v.begin(), v.end(), *p), v.end());
+    (void)hipFree(*p);
+    c->bytes -= bytes;
+    *p = nullptr;
+}
+
+// Replaces a tracked device buffer by one of `bytes` bytes (contents not
+// kept), checking the ctx's HBM budget with the old buffer given back.
+int dev_regrow(sjb_ctx *c, void **p, size_t old_bytes, size_t bytes) {
+    const uint64_t budget = c->opts.hbm_budget_bytes;
+    const uint64_t held = c->bytes + c->ext_bytes - (*p ? (old_bytes ? old_bytes : 256) : 0);
+    if (budget && held + bytes > budget) return SJB_E_BUDGET;
+    dev_free(c, p, old_bytes);
+    return dev_alloc(c, p, bytes);
+}
+
 int host_alloc(sjb_ctx *c, void **p, size_t bytes) {
     *p = nullptr;
     hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
@@ -69,11 +91,15 @@ int out_prepare(sjb_ctx *c, sjb_result_tuple_t *out, uint64_t cap, OutBuf *ob) {
     ob->dev = out;
     if (cap == 0 || out == nullptr || device_accessible(out)) return SJB_OK;
     if (c->out_stage_cap < cap) {
-        void *p = nullptr;
-        int rc = dev_alloc(c, &p, cap * sizeof(sjb_result_tuple_t));
-        if (rc) return rc;
+        // geometric growth: callers that size their buffer to each window's
+        // count would otherwise reallocate on every larger window
+        const uint64_t want = std::max<uint64_t>(cap, c->out_stage_cap + c->out_stage_cap / 2);
+        void *p = c->d_out_stage;
+        int rc = dev_regrow(c, &p, c->out_stage_cap * sizeof(sjb_result_tuple_t),
+                            want * sizeof(sjb_result_tuple_t));
         c->d_out_stage = static_cast<sjb_result_tuple_t *>(p);
-        c->out_stage_cap = cap;
+        c->out_stage_cap = c->d_out_stage ? want : 0;
+        if (rc) return rc;
     }
     ob->dev = c->d_out_stage;
     ob->host = out;

@@ -217,18 +217,13 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
     const bool c2 = (flags & SJB_F_C2) != 0;
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
     if (c->nlj_keys_cap < nL) {  // grow-only key buffer, owned by the ctx
-        if (c->d_nlj_keys) {
-            auto &v = c->dev_allocs;
-            v.erase(std::remove(v.begin(), v.end(), static_cast<void *>(c->d_nlj_keys)), v.end());
-            (void)hipFree(c->d_nlj_keys);
-            c->bytes -= (c->nlj_keys_cap + kNljKeySlack) * sizeof(uint32_t);
-            c->d_nlj_keys = nullptr;
-            c->nlj_keys_cap = 0;
-        }
         const uint64_t want = std::max<uint64_t>(nL, c->cap);
-        int rc = dev_alloc(c, (void **)&c->d_nlj_keys, (want + kNljKeySlack) * sizeof(uint32_t));
+        void *p = c->d_nlj_keys;
+        int rc = dev_regrow(c, &p, c->d_nlj_keys ? (c->nlj_keys_cap + kNljKeySlack) * sizeof(uint32_t) : 0,
+                            (want + kNljKeySlack) * sizeof(uint32_t));
+        c->d_nlj_keys = static_cast<uint32_t *>(p);
+        c->nlj_keys_cap = c->d_nlj_keys ? want : 0;
         if (rc) return rc;
-        c->nlj_keys_cap = want;
     }
     const uint32_t *lkeys = c->d_nlj_keys;
     k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys);
@@ -362,16 +357,29 @@ int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_t
     gy = (nL + l_split - 1) / l_split;
     auto *cnt = reinterpret_cast<unsigned long long *>(c->d_cursor) + 1;
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
-    int rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
-    if (rc) return rc;
-    k_nlj_bitmap<<<dim3(unsigned(gx), unsigned(gy)), kBmThreads, 0, c->stream>>>(
-        L, nL, R, nR, l_split, c->d_masks, c->mask_cap, cnt);
-    SJB_HIP_TRY(hipGetLastError());
+    // The number of (left tuple, 64-key word) masks with a match is a
+    // property of the window, not of the launch: when the first pass finds
+    // more than the buffer holds (it stores none past the end, but counts
+    // them all), the buffer grows to that count and the pass runs again.
     unsigned long long nm = 0;
-    SJB_HIP_TRY(hipMemcpyAsync(&nm, cnt, sizeof(nm), hipMemcpyDeviceToHost, c->stream));
-    SJB_HIP_TRY(hipStreamSynchronize(c->stream));
-    *n_masks_host = nm;
-    if (nm > c->mask_cap) return SJB_E_CAPACITY;
+    for (int pass = 0;; pass++) {
+        int rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
+        if (rc) return rc;
+        k_nlj_bitmap<<<dim3(unsigned(gx), unsigned(gy)), kBmThreads, 0, c->stream>>>(
+            L, nL, R, nR, l_split, c->d_masks, c->mask_cap, cnt);
+        SJB_HIP_TRY(hipGetLastError());
+        SJB_HIP_TRY(hipMemcpyAsync(&nm, cnt, sizeof(nm), hipMemcpyDeviceToHost, c->stream));
+        SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+        *n_masks_host = nm;
+        if (nm <= c->mask_cap) break;
+        if (pass > 0) return SJB_E_CAPACITY;  // cannot happen: the count is deterministic
+        const uint64_t want = std::max<uint64_t>(nm, c->mask_cap + c->mask_cap / 2);
+        void *p = c->d_masks;
+        rc = dev_regrow(c, &p, 3 * c->mask_cap * sizeof(uint64_t), 3 * want * sizeof(uint64_t));
+        c->d_masks = static_cast<uint64_t *>(p);
+        c->mask_cap = c->d_masks ? want : 0;
+        if (rc) return rc;
+    }
     const int blocks = 512;
     k_bitmap_decode<<<blocks, 256, 0, c->stream>>>(c->d_masks, cnt, L, R, out, out_cap, cur,
                                                    c->d_partials);

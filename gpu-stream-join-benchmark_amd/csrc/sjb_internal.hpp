@@ -266,6 +266,8 @@ namespace sjb {
 // allocation helpers (tracked; freed by sjb_close)
 int dev_alloc(sjb_ctx *c, void **p, size_t bytes);
 int host_alloc(sjb_ctx *c, void **p, size_t bytes);
+void dev_free(sjb_ctx *c, void **p, size_t bytes);
+int dev_regrow(sjb_ctx *c, void **p, size_t old_bytes, size_t bytes);
 
 // ---- launchers (partition.hip) ----
 int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested);

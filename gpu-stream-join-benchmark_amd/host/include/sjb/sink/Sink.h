@@ -1,8 +1,12 @@
 // Sink — result channel shared by the executors.  Same public surface as
 // the reference's Sink (include/sjb/sink/Sink.h:23-126): match counter,
-// possible matches, per-window progress, phase-time totals, a result-tuple
-// ring.  The ring lives in host memory (the C ABI stages device output);
-// `counterGPU` is an ordinary counter that the Atomic executors add to.
+// possible matches, per-window progress, phase-time totals and the public
+// members sinkBuffer (a ring of sinkBufferSize result tuples), counterGPU
+// (one counter that the Atomic executors add to; callers reset it with
+// counterGPU[0] = 0, as src/experiment/Runner.cpp:44 does).  Both are
+// ordinary host memory here: the C ABI stages device output into them.
+// addToStatistics (a no-op in the reference, src/sjb/sink/Sink.cpp:65-67)
+// accumulates the per-window checksums that ExecutionStatistic carries.
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -22,7 +26,9 @@ struct Progress {
 class Sink {
 public:
     explicit Sink(uint64_t sinkBufferSize = 1ull << 20);
-    ~Sink();
+    virtual ~Sink();
+    Sink(const Sink &) = delete;
+    Sink &operator=(const Sink &) = delete;
 
     void addToCounter(uint64_t currentMatches, bool isWindowClosed);
     void addToPossibleMatches(uint64_t count);
@@ -33,7 +39,8 @@ public:
     void addToSortTime(uint64_t time);
     void addToMergeTime(uint64_t time);
     void addToExecutedWindows(uint64_t n);
-    void addToChecksums(uint64_t c1, uint64_t c2);
+    // adds the statistic's checksums to the run's totals (getChecksum1/2)
+    void addToStatistics(ExecutionStatistic statistic);
     uint64_t getSinkBufferSize() const;
     uint64_t getAverageSortTime();
     uint64_t getAverageMergeTime();
@@ -44,15 +51,18 @@ public:
     // store one result tuple (bitmap decode path of the reference)
     void incrementCounterAndStore(uint32_t lkey, uint32_t rkey, uint32_t lVal, uint32_t rVal,
                                   uint64_t lTs, uint64_t rTs);
-    // account `tupleCount` results already written into sinkBuffer
+    // count `tupleCount` results and copy the first min(tupleCount,
+    // sinkBufferSize) of them to the start of sinkBuffer
     void incrementCounterAndStore(ResultTuple *resultTuple, uint64_t tupleCount);
     const std::atomic<uint64_t> &getTotalExecutedWindows() const;
+    // run totals of the checksums handed in through addToStatistics
     uint64_t getChecksum1() const { return checksum1.load(); }
     uint64_t getChecksum2() const { return checksum2.load(); }
 
-    std::vector<ResultTuple> sinkBuffer;
+    // Sink ring buffer of results (include/sjb/sink/Sink.h:92-95)
+    ResultTuple *sinkBuffer = nullptr;
     uint64_t sinkBufferSize;
-    std::atomic<uint64_t> counterGPU{0};
+    unsigned long long *counterGPU = nullptr;
 
 private:
     std::atomic<uint64_t> counter{0};

@@ -1,9 +1,11 @@
 // Source (include/sjb/source/Source.hpp, src/sjb/source/Source.cpp:12-34):
 // pushes batch i of the left stream then batch i of the right stream to the
 // windowing, for i over count / bufferSize batches.  sourceThreads == 1 keeps
-// the strict L0,R0,L1,R1,... order (deterministic windows); more threads
-// pull batch indexes from a shared counter like the reference's
-// parallel_for.
+// the strict L0,R0,L1,R1,... order; more threads pull batch indexes from a
+// shared counter like the reference's parallel_for.  Every batch is pushed
+// with its index (BaseWindowing::onIncomingLeft(tuples, i)), so the
+// windowing places it by stream position and windows are the same for any
+// thread count.
 #pragma once
 #include <cstdint>
 #include <memory>

@@ -22,6 +22,20 @@ public:
     virtual ~BaseWindowing() = default;
     virtual void onIncomingLeft(Tuple *tupleBuffer) = 0;
     virtual void onIncomingRight(Tuple *tupleBuffer) = 0;
+    // The same, for batch `batchIndex` of its stream (0, 1, 2, ... in
+    // stream order).  With the index a windowing can place the batch by its
+    // position in the stream rather than by arrival order, so windows do not
+    // depend on how concurrent source threads interleave (the reference
+    // windows by arrival order, include/sjb/windowing/lazy/CountBased.cuh:112-125).
+    // Callers that do not know the index use the calls above.
+    virtual void onIncomingLeft(Tuple *tupleBuffer, uint64_t batchIndex) {
+        (void)batchIndex;
+        onIncomingLeft(tupleBuffer);
+    }
+    virtual void onIncomingRight(Tuple *tupleBuffer, uint64_t batchIndex) {
+        (void)batchIndex;
+        onIncomingRight(tupleBuffer);
+    }
     virtual void onEndOfStream() = 0;
     const std::string &getAlgorithmName() const { return algorithmName; }
 

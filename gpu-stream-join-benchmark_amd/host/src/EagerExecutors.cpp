@@ -11,6 +11,16 @@ namespace Windowing {
 namespace Eager {
 namespace Executor {
 
+// The eager execute returns only the batch's match count (the reference's
+// signature); its checksums reach the sink as an ExecutionStatistic.
+static void addChecksums(Sink &sink, uint64_t c1, uint64_t c2, bool c2Valid) {
+    ExecutionStatistic es;
+    es.checksum1 = c1;
+    es.checksum2 = c2;
+    es.checksum2Valid = c2Valid;
+    sink.addToStatistics(es);
+}
+
 GPU_Eager::GPU_Eager(int algorithm, int method, uint64_t numDistinctKeys, uint64_t windowSize,
                      uint64_t batchSize, uint32_t flags)
     : algorithm(algorithm), method(method), numDistinctKeys(numDistinctKeys), windowSize(windowSize),
@@ -32,18 +42,18 @@ uint64_t GPU_Eager::execute(Tuple *tupleBuffer, bool isLeftSide, uint64_t, Sink 
         sjbCheck(sjb_eager_execute(ctx, algorithm, method, flags, as_abi(tupleBuffer), batchSize,
                                    isLeftSide ? 1 : 0, nullptr, 0, &st),
                  "sjb_eager_execute");
-        sink.addToChecksums(st.checksum1, st.c2_valid ? st.checksum2 : 0);
+        addChecksums(sink, st.checksum1, st.checksum2, st.c2_valid != 0);
         return st.result_count;
     }
     if (method == SJB_OUT_ATOMIC) {
         // pairs go round the sink ring; the count goes to counterGPU
         sjbCheck(sjb_eager_execute(ctx, algorithm, method, flags, as_abi(tupleBuffer), batchSize,
                                    isLeftSide ? 1 : 0,
-                                   reinterpret_cast<sjb_result_tuple_t *>(sink.sinkBuffer.data()),
+                                   reinterpret_cast<sjb_result_tuple_t *>(sink.sinkBuffer),
                                    sink.getSinkBufferSize(), &st),
                  "sjb_eager_execute");
-        sink.counterGPU.fetch_add(st.result_count);
-        sink.addToChecksums(st.checksum1, st.checksum2);
+        __atomic_fetch_add(sink.counterGPU, (unsigned long long)st.result_count, __ATOMIC_RELAXED);
+        addChecksums(sink, st.checksum1, st.checksum2, true);
         return 0;
     }
     // Bitmap: decoded pairs are handed to the sink (the reference decodes
@@ -64,7 +74,7 @@ uint64_t GPU_Eager::execute(Tuple *tupleBuffer, bool isLeftSide, uint64_t, Sink 
         sink.incrementCounterAndStore(scratch.data() + written, n);
         written += n;
     }
-    sink.addToChecksums(st.checksum1, st.checksum2);
+    addChecksums(sink, st.checksum1, st.checksum2, true);
     return 0;
 }
 
@@ -87,7 +97,7 @@ GPU_HJ_PK::~GPU_HJ_PK() {
 uint64_t GPU_HJ_PK::execute(Tuple *tupleBuffer, bool isLeftSide, uint64_t, Sink &sink) {
     sjb_stats_t st{};
     sjbCheck(sjb_pk_execute(ctx, as_abi(tupleBuffer), batchSize, isLeftSide ? 1 : 0, &st), "sjb_pk_execute");
-    sink.addToChecksums(st.checksum1, st.checksum2);
+    addChecksums(sink, st.checksum1, st.checksum2, true);
     return st.result_count;
 }
 
@@ -133,7 +143,7 @@ uint64_t CPU_HJ::execute(Tuple *tupleBuffer, bool isLeftSide, uint64_t, Sink &si
             c1 += sjb_hash_a(l.key, l.val) * sjb_hash_b(r.key, r.val);
         }
     }
-    sink.addToChecksums(c1, 0);
+    addChecksums(sink, c1, 0, false);
     return count;
 }
 

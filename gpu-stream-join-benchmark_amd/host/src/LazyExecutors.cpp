@@ -47,8 +47,8 @@ ExecutionStatistic GPU_Lazy::execute(Tuple *leftRingBuffer, Tuple *rightRingBuff
     } else if (method == SJB_OUT_ATOMIC) {
         // pairs wrap around the sink ring; the count goes to counterGPU
         // (the reference's Atomic executors add to the pinned counterGPU)
-        sjbCheck(call(SJB_OUT_ATOMIC, sink.sinkBuffer.data(), sink.getSinkBufferSize()), "sjb_lazy_execute");
-        sink.counterGPU.fetch_add(st.result_count);
+        sjbCheck(call(SJB_OUT_ATOMIC, sink.sinkBuffer, sink.getSinkBufferSize()), "sjb_lazy_execute");
+        __atomic_fetch_add(sink.counterGPU, (unsigned long long)st.result_count, __ATOMIC_RELAXED);
     } else {
         // CountKernel / EstimatedSelectivity / Bitmap: write every pair, then
         // hand them to the sink in sink-sized chunks, like the reference
@@ -72,7 +72,6 @@ ExecutionStatistic GPU_Lazy::execute(Tuple *leftRingBuffer, Tuple *rightRingBuff
             written += n;
         }
     }
-    sink.addToChecksums(st.checksum1, st.c2_valid ? st.checksum2 : 0);
     if (algorithm == SJB_ALG_SMJ) {
         es.sortTime = st.sort_time_us;
         es.mergeTime = st.merge_time_us;
@@ -83,6 +82,7 @@ ExecutionStatistic GPU_Lazy::execute(Tuple *leftRingBuffer, Tuple *rightRingBuff
     es.checksum1 = st.checksum1;
     es.checksum2 = st.checksum2;
     es.checksum2Valid = st.c2_valid != 0;
+    sink.addToStatistics(es);
     return es;
 }
 
@@ -144,7 +144,7 @@ ExecutionStatistic CPU_HJ::execute(Tuple *leftRingBuffer, Tuple *rightRingBuffer
     }
     es.buildTime = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count());
     es.probeTime = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(t2 - t1).count());
-    sink.addToChecksums(es.checksum1, 0);
+    sink.addToStatistics(es);
     return es;
 }
 
@@ -198,7 +198,7 @@ ExecutionStatistic CPU_SMJ::execute(Tuple *leftRingBuffer, Tuple *rightRingBuffe
     const auto t2 = std::chrono::high_resolution_clock::now();
     es.sortTime = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count());
     es.mergeTime = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(t2 - t1).count());
-    sink.addToChecksums(es.checksum1, 0);
+    sink.addToStatistics(es);
     return es;
 }
 
@@ -218,7 +218,7 @@ ExecutionStatistic CPU_NLJ::execute(Tuple *leftRingBuffer, Tuple *rightRingBuffe
                         es.checksum1 += sjb_hash_a(leftRingBuffer[a].key, leftRingBuffer[a].val) *
                                         sjb_hash_b(rightRingBuffer[b].key, rightRingBuffer[b].val);
                     }
-    sink.addToChecksums(es.checksum1, 0);
+    sink.addToStatistics(es);
     return es;
 }
 

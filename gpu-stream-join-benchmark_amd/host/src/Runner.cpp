@@ -36,6 +36,7 @@ Runner::RunResult Runner::runExperimentWithConfig(ExperimentConfig config, bool 
 
     Source source(d.dataSize, d.batchSize);
     Sink sink;
+    sink.counterGPU[0] = 0;  // Runner.cpp:44
     auto windowing = Windowing::WindowingFactory::create(config.algorithmConfig, config.dataConfig,
                                                          config.queryConfig, config.engineConfig, sink);
     const auto t0 = std::chrono::high_resolution_clock::now();

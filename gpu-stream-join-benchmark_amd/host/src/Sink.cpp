@@ -2,10 +2,24 @@
 #include "sjb/sink/Sink.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <new>
 
-Sink::Sink(uint64_t sinkBufferSize) : sinkBuffer(sinkBufferSize), sinkBufferSize(sinkBufferSize) {}
-Sink::~Sink() = default;
+Sink::Sink(uint64_t sinkBufferSize) : sinkBufferSize(sinkBufferSize) {
+    sinkBuffer = static_cast<ResultTuple *>(std::calloc(sinkBufferSize ? sinkBufferSize : 1, sizeof(ResultTuple)));
+    counterGPU = static_cast<unsigned long long *>(std::calloc(1, sizeof(unsigned long long)));
+    if (!sinkBuffer || !counterGPU) {
+        std::free(sinkBuffer);
+        std::free(counterGPU);
+        throw std::bad_alloc();
+    }
+}
+
+Sink::~Sink() {
+    std::free(sinkBuffer);
+    std::free(counterGPU);
+}
 
 void Sink::addToCounter(uint64_t currentMatches, bool isWindowClosed) {
     const uint64_t now = counter.fetch_add(currentMatches) + currentMatches;
@@ -15,7 +29,9 @@ void Sink::addToCounter(uint64_t currentMatches, bool isWindowClosed) {
     progress.push_back(Progress{now, isWindowClosed, uint64_t(us)});
 }
 
-uint64_t Sink::getMatchCount(bool useGPUCounter) { return useGPUCounter ? counterGPU.load() : counter.load(); }
+uint64_t Sink::getMatchCount(bool useGPUCounter) {
+    return useGPUCounter ? __atomic_load_n(counterGPU, __ATOMIC_RELAXED) : counter.load();
+}
 const std::atomic<uint64_t> &Sink::getPossibleMatches() const { return possibleMatches; }
 void Sink::addToPossibleMatches(uint64_t c) { possibleMatches.fetch_add(c); }
 void Sink::addToBuildTime(uint64_t t) { totalBuildTime.fetch_add(t); }
@@ -23,9 +39,9 @@ void Sink::addToProbeTime(uint64_t t) { totalProbeTime.fetch_add(t); }
 void Sink::addToSortTime(uint64_t t) { totalSortTime.fetch_add(t); }
 void Sink::addToMergeTime(uint64_t t) { totalMergeTime.fetch_add(t); }
 void Sink::addToExecutedWindows(uint64_t n) { totalExecutedWindows.fetch_add(n); }
-void Sink::addToChecksums(uint64_t c1, uint64_t c2) {
-    checksum1.fetch_add(c1);
-    checksum2.fetch_add(c2);
+void Sink::addToStatistics(ExecutionStatistic statistic) {
+    checksum1.fetch_add(statistic.checksum1);
+    checksum2.fetch_add(statistic.checksum2Valid ? statistic.checksum2 : 0);
 }
 uint64_t Sink::getSinkBufferSize() const { return sinkBufferSize; }
 const std::atomic<uint64_t> &Sink::getTotalExecutedWindows() const { return totalExecutedWindows; }
@@ -64,5 +80,5 @@ void Sink::incrementCounterAndStore(uint32_t lkey, uint32_t rkey, uint32_t lVal,
 void Sink::incrementCounterAndStore(ResultTuple *resultTuple, uint64_t tupleCount) {
     counter.fetch_add(tupleCount);
     const uint64_t n = std::min(tupleCount, sinkBufferSize);
-    if (n && resultTuple != sinkBuffer.data()) std::memcpy(sinkBuffer.data(), resultTuple, n * sizeof(ResultTuple));
+    if (n && resultTuple != sinkBuffer) std::memcpy(sinkBuffer, resultTuple, n * sizeof(ResultTuple));
 }

@@ -14,8 +14,8 @@ void Source::start(Tuple *leftTuples, Tuple *rightTuples, uint64_t sourceThreads
     const uint64_t nb = bufferSize ? count / bufferSize : 0;
     if (sourceThreads <= 1) {
         for (uint64_t i = 0; i < nb; i++) {
-            windowing->onIncomingLeft(leftTuples + i * bufferSize);
-            windowing->onIncomingRight(rightTuples + i * bufferSize);
+            windowing->onIncomingLeft(leftTuples + i * bufferSize, i);
+            windowing->onIncomingRight(rightTuples + i * bufferSize, i);
         }
     } else {
         std::atomic<uint64_t> next{0};
@@ -26,8 +26,8 @@ void Source::start(Tuple *leftTuples, Tuple *rightTuples, uint64_t sourceThreads
             th.emplace_back([&] {
                 try {
                     for (uint64_t i = next.fetch_add(1); i < nb; i = next.fetch_add(1)) {
-                        windowing->onIncomingLeft(leftTuples + i * bufferSize);
-                        windowing->onIncomingRight(rightTuples + i * bufferSize);
+                        windowing->onIncomingLeft(leftTuples + i * bufferSize, i);
+                        windowing->onIncomingRight(rightTuples + i * bufferSize, i);
                     }
                 } catch (...) {
                     std::lock_guard<std::mutex> g(errMtx);

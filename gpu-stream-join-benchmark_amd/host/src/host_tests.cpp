@@ -102,6 +102,83 @@ static void test_lazy_windows_canonical() {
     CHECK(sink.getPossibleMatches().load() == (n / W) * W * W);
 }
 
+// Windows must not depend on how concurrent source threads interleave:
+// 32 source threads, several pooled executors running windows at once and a
+// ring of only two windows per side, five runs; every run must give the
+// canonical per-window totals (count and C1 over L[wW,(w+1)W) x R[wW,(w+1)W)).
+static void test_lazy_windows_concurrent_sources() {
+    using namespace Windowing::Lazy::Executor;
+    const uint64_t B = 16, W = 8 * B, nw = 40, n = nw * W;
+    auto L = rnd(n, 64, 13), R = rnd(n, 64, 14);
+    uint64_t expect = 0, expect_c1 = 0;
+    for (uint64_t o = 0; o < n; o += W) {
+        auto b = brute(L.data() + o, W, R.data() + o, W);
+        expect += b.first;
+        expect_c1 += b.second;
+    }
+    // the same tuples split into batches in push order would give another
+    // total: the check below can tell canonical windows from arrival order
+    uint64_t shifted = 0;
+    for (uint64_t o = B; o + W <= n; o += W) shifted += brute(L.data() + o, W, R.data() + o - B, W).first;
+    CHECK(shifted != expect);
+    for (int run = 0; run < 5; run++) {
+        Sink sink;
+        std::vector<std::shared_ptr<BaseLazyExecutor>> js;
+        for (int j = 0; j < 3; j++) js.push_back(std::make_shared<CPU_HJ>(1000, B, 1, W));
+        auto w = std::make_shared<Windowing::Lazy::CountBased>(sink, 2 * W, 1u << 30, W, B, js, "lazy_cpu_hj_nm");
+        Source(n, B).start(L.data(), R.data(), 32, w);
+        CHECK(sink.getMatchCount() == expect);
+        CHECK(sink.getChecksum1() == expect_c1);
+        CHECK(sink.getTotalExecutedWindows().load() == nw);
+    }
+}
+
+// Eager windows with concurrent sources: batches are joined in stream order
+// (L0,R0,L1,R1,...), so the totals equal the single-threaded run.
+static void test_eager_concurrent_sources() {
+    using namespace Windowing::Eager;
+    const uint64_t B = 16, W = 4 * B, n = 24 * W;
+    auto L = rnd(n, 30, 15), R = rnd(n, 30, 16);
+    uint64_t expect = 0;
+    for (uint64_t o = 0; o < n; o += W) expect += brute(L.data() + o, W, R.data() + o, W).first;
+    for (int run = 0; run < 3; run++) {
+        Sink sink;
+        auto ex = std::make_shared<Executor::CPU_HJ>(W, 1000, B);
+        auto w = std::make_shared<CountBased>(sink, W, B, ex, "eager_cpu_hj_nm", CANONICAL);
+        Source(n, B).start(L.data(), R.data(), 16, w);
+        CHECK(sink.getMatchCount() == expect);
+    }
+}
+
+// Reference callers use the Sink's public members directly
+// (src/experiment/Runner.cpp:44 resets counterGPU[0]; the Atomic executors
+// add to it and write into sinkBuffer).
+static void test_sink_reference_members() {
+    Sink sink(8);
+    sink.counterGPU[0] = 0;
+    sink.counterGPU[0] += 5;
+    CHECK(sink.getMatchCount(true) == 5 && sink.getMatchCount(false) == 0);
+    ResultTuple *ring = sink.sinkBuffer;
+    CHECK(ring != nullptr && sink.getSinkBufferSize() == 8 && sink.sinkBufferSize == 8);
+    ring[7].lkey = 42;
+    CHECK(sink.sinkBuffer[7].lkey == 42);
+    ResultTuple rows[3] = {};
+    rows[0].lkey = 1;
+    rows[2].rVal = 9;
+    sink.incrementCounterAndStore(rows, 3);
+    CHECK(sink.getMatchCount() == 3 && sink.sinkBuffer[0].lkey == 1 && sink.sinkBuffer[2].rVal == 9);
+    sink.incrementCounterAndStore(11, 12, 13, 14, 15, 16);  // position 3 of the ring
+    CHECK(sink.getMatchCount() == 4 && sink.sinkBuffer[3].lkey == 11 && sink.sinkBuffer[3].rTs == 16);
+    ExecutionStatistic es;
+    es.checksum1 = 7;
+    es.checksum2 = 100;
+    es.checksum2Valid = false;
+    sink.addToStatistics(es);
+    es.checksum2Valid = true;
+    sink.addToStatistics(es);
+    CHECK(sink.getChecksum1() == 14 && sink.getChecksum2() == 100);
+}
+
 static void test_eager_close_rules() {
     using namespace Windowing::Eager;
     const uint64_t B = 16, W = 3 * B, n = 4 * W;
@@ -205,6 +282,9 @@ int main() {
     test_yaml();
     test_cpu_executors_agree();
     test_lazy_windows_canonical();
+    test_lazy_windows_concurrent_sources();
+    test_eager_concurrent_sources();
+    test_sink_reference_members();
     test_eager_close_rules();
     test_factory();
     test_misc();

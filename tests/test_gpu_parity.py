@@ -29,8 +29,6 @@ def run_count(ctx, torch, alg, R, S, flags=F_C2):
 @pytest.mark.parametrize("size", SIZES)
 def test_count_checksums(ctx, torch_mod, orc, alg, key_range, size):
     nL, nR = size
-    if alg == NLJ and nL * nR > 1 << 32:
-        pytest.skip("NLJ size")
     print(f"case alg={alg} range={key_range} size={size}", flush=True)
     R, S = make(nL, key_range, 1000 + nL), make(nR, key_range, 2000 + nR)
     ref = np_join(R, S) if nL * nR < 1 << 26 or key_range > 1 << 12 else None

@@ -30,7 +30,7 @@ def run_driver(tmp_path, yaml_text, name="out.csv", env=None):
 
 
 def yaml_cfg(alg, dev, prog, method, window_mb=1, batch_kb=128, keys=1000, theta=0.0, data=1 << 17,
-             par="INTRA", pk="No"):
+             par="INTRA", pk="No", source_threads=1):
     return f"""algTypes: [ "{alg}" ]
 deviceTypes: [ "{dev}" ]
 progressivenessModes: [ "{prog}" ]
@@ -46,7 +46,7 @@ parStrats: [ "{par}" ]
 dataSizeTuples: {data}
 rateSearch: "No"
 extendedCsv: "Yes"
-sourceThreads: 1
+sourceThreads: {source_threads}
 usePK: "{pk}"
 """
 
@@ -60,6 +60,21 @@ def test_host_unit_binary():
                        env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "0 failures" in p.stdout
+
+
+@pytest.mark.parametrize("san", ["tsan", "asan"])
+def test_host_sanitizers(san):
+    """host_tests built with ThreadSanitizer (32 source threads into the
+    windowing, pooled executors) and with AddressSanitizer + UBSan."""
+    p = subprocess.run(["make", "-s", "-C", os.path.join(PKG, "host"), f"{BIN}/host_tests_{san}"],
+                       capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stderr[-3000:]
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66",
+               ASAN_OPTIONS="detect_leaks=1 exitcode=66", UBSAN_OPTIONS="halt_on_error=1")
+    p = subprocess.run([os.path.join(BIN, f"host_tests_{san}")], capture_output=True, text=True, timeout=600,
+                       env=env)
+    assert p.returncode == 0, (p.stdout + p.stderr)[-4000:]
+    assert "0 failures" in p.stdout and "Sanitizer" not in p.stderr
 
 
 def test_driver_rejects_bad_yaml(tmp_path):
@@ -100,6 +115,18 @@ def test_driver_cpu_lazy(tmp_path, sjbmod, orc, alg):
     assert int(r["checksum1"]) == c1
     sel = 100.0 * cnt / ((data // (window_mb * 65536)) * (window_mb * 65536) ** 2)
     assert abs(float(r["selectivity"]) - sel) < 1e-6 * max(sel, 1)
+
+
+@pytest.mark.parametrize("prog", ["LAZY", "EAGER"])
+def test_driver_cpu_concurrent_sources(tmp_path, sjbmod, orc, prog):
+    # 32 source threads (the reference driver's default): windows are
+    # placed by stream position, so the result is the canonical one
+    data = 1 << 18
+    for par in ("INTRA", "INTER"):
+        rows = run_driver(tmp_path, yaml_cfg("HJ", "CPU", prog, "NoOutput", data=data, par=par,
+                                             source_threads=32), name=f"{par}.csv")
+        cnt, c1 = expected_lazy(sjbmod, orc, data, 65536, 1000)
+        assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1, par
 
 
 def test_driver_cpu_eager(tmp_path, sjbmod, orc):
@@ -150,4 +177,28 @@ def test_driver_gpu_pk(tmp_path, sjbmod, orc):
                       env={"SJB_HAVE_GPU": "1"})
     assert rows[0]["algorithm_name"] == "eager_gpu_hj_nm_pk"
     cnt, c1 = expected_lazy(sjbmod, orc, data, 65536, 5000)
+    assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,method", [("LAZY", "NoOutput"), ("LAZY", "Atomic"), ("EAGER", "NoOutput")])
+def test_driver_gpu_concurrent_sources(tmp_path, sjbmod, orc, prog, method):
+    print(f"case driver sources=32 {prog} {method}", flush=True)
+    data = 1 << 19
+    rows = run_driver(tmp_path, yaml_cfg("HJ", "GPU", prog, method, data=data, keys=5000, par="INTER",
+                                         source_threads=32), env={"SJB_HAVE_GPU": "1"})
+    cnt, c1 = expected_lazy(sjbmod, orc, data, 65536, 5000)
+    assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,window_mb,data", [("LAZY", 1, 1 << 17), ("EAGER", 5, 1 << 19)])
+def test_driver_gpu_bitmap_dense(tmp_path, sjbmod, orc, prog, window_mb, data):
+    # 1000 distinct keys (the reference configs' value): a 1 MB lazy window
+    # has ~4.2M (left tuple, 64-key word) masks with a match, a 5 MB eager
+    # window ~1.6M per batch pass; the mask buffer grows to what is needed
+    print(f"case driver bitmap D=1000 {prog} {window_mb}MB", flush=True)
+    rows = run_driver(tmp_path, yaml_cfg("NLJ", "GPU", prog, "Bitmap", window_mb=window_mb, data=data, keys=1000),
+                      env={"SJB_HAVE_GPU": "1"})
+    cnt, c1 = expected_lazy(sjbmod, orc, data, window_mb * 65536, 1000)
     assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
