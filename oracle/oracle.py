@@ -56,6 +56,8 @@ def lib():
             "or_eager_execute": (_i, [_p, _p, _u64, _i, _i, _R, _p, _u64]),
             "or_sliding": (_i, [_p, _p, _u64, _u64, _u64, _p]),
             "or_bench_lazy_hj": (ctypes.c_double, [_p, _p, _u64, _u64, _u64, _i, _R]),
+            "or_bench_lazy_hj_inter": (ctypes.c_double, [_p, _p, _u64, _u64, _u64, _i, _R]),
+            "or_bench_slide": (ctypes.c_double, [_p, _p, _p, _p, _u64, _u64, _p]),
             "or_mlp_grad": (ctypes.c_double, [_p, _p, _p, _u32, _p]),
             "or_sliding_var": (_i, [_p, _p, _p, _p, _u64, _u64, _p]),
             "or_mlp_loss": (ctypes.c_double, [_p, _p, _p, _u32]),
@@ -147,10 +149,24 @@ def sliding_var(R, endR, S, endS, W):
     return [(res[i].count, res[i].c1) for i in range(n)]
 
 
-def bench_lazy_hj(R, S, W, D, threads):
+def bench_slide(R, endR, S, endS, W):
+    """Seconds and per-event (count, C1) of the incremental CPU sliding join."""
+    endR = np.ascontiguousarray(endR, dtype=np.uint64)
+    endS = np.ascontiguousarray(endS, dtype=np.uint64)
+    n = len(endR)
+    res = (Result * max(n, 1))()
+    secs = lib().or_bench_slide(_ptr(R), endR.ctypes.data, _ptr(S), endS.ctypes.data, n, W,
+                                ctypes.cast(res, ctypes.c_void_p))
+    return secs, [(res[i].count, res[i].c1) for i in range(n)]
+
+
+def bench_lazy_hj(R, S, W, D, threads, inter=False):
+    """Seconds and totals of the timed CPU hash join over the tumbling
+    windows of R and S: INTRA (1 joiner x `threads` probe threads) or INTER
+    (`threads` joiners taking whole windows)."""
     tot = Result()
-    secs = lib().or_bench_lazy_hj(_ptr(R), _ptr(S), min(len(R), len(S)), W, D, threads,
-                                  ctypes.byref(tot))
+    fn = lib().or_bench_lazy_hj_inter if inter else lib().or_bench_lazy_hj
+    secs = fn(_ptr(R), _ptr(S), min(len(R), len(S)), W, D, threads, ctypes.byref(tot))
     return secs, tot
 
 

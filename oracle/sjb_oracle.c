@@ -327,15 +327,15 @@ double or_bench_lazy_hj(const or_tuple *R, const or_tuple *S, uint64_t n, uint64
                         int threads, or_result *total) {
     memset(total, 0, sizeof(*total));
     if (threads < 1) threads = 1;
-    if (threads > 256) threads = 256;
+    if (threads > 1024) threads = 1024;
     uint64_t *occ = malloc(D * sizeof(uint64_t));
     uint64_t *prefix = malloc(D * sizeof(uint64_t));
     uint64_t *ins = malloc(D * sizeof(uint64_t));
     or_tuple *store = malloc(W * sizeof(or_tuple));
-    pthread_t tid[256];
-    probe_arg args[256];
-    if (!occ || !prefix || !ins || !store) {
-        free(occ); free(prefix); free(ins); free(store);
+    pthread_t *tid = malloc((size_t)threads * sizeof(pthread_t));
+    probe_arg *args = malloc((size_t)threads * sizeof(probe_arg));
+    if (!occ || !prefix || !ins || !store || !tid || !args) {
+        free(occ); free(prefix); free(ins); free(store); free(tid); free(args);
         return -1.0;
     }
     struct timespec t0, t1;
@@ -436,4 +436,191 @@ double or_mlp_grad(const double *P, const double *x, const double *t, uint32_t n
         }
     }
     return loss / n;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Timed CPU baseline, INTER decomposition: `threads` joiners, each with   */
+/* its own single-threaded CPU_HJ (executable/runExperiment.cpp:167-170:   */
+/* INTER = joinerThreads executors, one probe thread each), take whole     */
+/* tumbling windows round-robin (window w to joiner w mod threads).        */
+/* ---------------------------------------------------------------------- */
+typedef struct joiner_arg {
+    const or_tuple *R, *S;
+    uint64_t n, W, D;
+    int id, threads;
+    or_result r;
+    int fail;
+} joiner_arg;
+
+static void *joiner_worker(void *p) {
+    joiner_arg *a = p;
+    memset(&a->r, 0, sizeof(a->r));
+    const uint64_t D = a->D, W = a->W;
+    uint64_t *occ = malloc(D * sizeof(uint64_t));
+    uint64_t *prefix = malloc(D * sizeof(uint64_t));
+    uint64_t *ins = malloc(D * sizeof(uint64_t));
+    or_tuple *store = malloc((W ? W : 1) * sizeof(or_tuple));
+    if (!occ || !prefix || !ins || !store) {
+        a->fail = 1;
+        free(occ); free(prefix); free(ins); free(store);
+        return NULL;
+    }
+    uint64_t w = 0;
+    for (uint64_t o = 0; o + W <= a->n; o += W, w++) {
+        if ((int)(w % (uint64_t)a->threads) != a->id) continue;
+        const or_tuple *r = a->R + o, *s = a->S + o;
+        memset(occ, 0, D * sizeof(uint64_t));
+        memset(ins, 0, D * sizeof(uint64_t));
+        for (uint64_t i = 0; i < W; i++) occ[r[i].key % D]++;
+        uint64_t run = 0;
+        for (uint64_t h = 0; h < D; h++) {
+            prefix[h] = run;
+            run += occ[h];
+        }
+        for (uint64_t i = 0; i < W; i++) {
+            uint64_t h = r[i].key % D;
+            store[prefix[h] + ins[h]++] = r[i];
+        }
+        for (uint64_t i = 0; i < W; i++) {
+            uint64_t h = s[i].key % D;
+            for (uint64_t j = prefix[h]; j < prefix[h] + occ[h]; j++)
+                if (store[j].key == s[i].key) {
+                    a->r.count++;
+                    a->r.c1 += or_hash_a(store[j].key, store[j].val) * or_hash_b(s[i].key, s[i].val);
+                    a->r.c2 += or_pair_hash(s[i].key, store[j].val, s[i].val);
+                }
+        }
+    }
+    free(occ); free(prefix); free(ins); free(store);
+    return NULL;
+}
+
+double or_bench_lazy_hj_inter(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uint64_t D,
+                              int threads, or_result *total) {
+    memset(total, 0, sizeof(*total));
+    if (threads < 1) threads = 1;
+    if (threads > 1024) threads = 1024;
+    if (D == 0 || W == 0) return -1.0;
+    pthread_t *tid = malloc((size_t)threads * sizeof(pthread_t));
+    joiner_arg *args = calloc((size_t)threads, sizeof(joiner_arg));
+    if (!tid || !args) {
+        free(tid); free(args);
+        return -1.0;
+    }
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < threads; t++) {
+        args[t].R = R;
+        args[t].S = S;
+        args[t].n = n;
+        args[t].W = W;
+        args[t].D = D;
+        args[t].id = t;
+        args[t].threads = threads;
+        pthread_create(&tid[t], NULL, joiner_worker, &args[t]);
+    }
+    int fail = 0;
+    for (int t = 0; t < threads; t++) {
+        pthread_join(tid[t], NULL);
+        fail |= args[t].fail;
+        total->count += args[t].r.count;
+        total->c1 += args[t].r.c1;
+        total->c2 += args[t].r.c2;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(tid); free(args);
+    if (fail) return -1.0;
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Timed CPU baseline of the sliding-window join (no reference             */
+/* counterpart: the reference windows are tumbling).  The same exact       */
+/* incremental method as the device path, single-threaded: per side a     */
+/* hash map key -> (tuples in the window, sum of A or B); after every      */
+/* event the window is the last W tuples of each side and                  */
+/*   J(L',R') = J(L,R) - J(Lout,R) - J(Lkeep,Rout) + J(Lin,Rkeep) + J(L',Rin) */
+/* for count and C1.  res[i] is the window result after event i.           */
+/* ---------------------------------------------------------------------- */
+typedef struct slide_slot {
+    uint32_t key;
+    uint32_t used;
+    uint64_t cnt;
+    uint64_t sum;
+} slide_slot;
+
+static slide_slot *slide_find(slide_slot *t, uint64_t mask, uint32_t key) {
+    uint64_t h = (or_mix64(key) >> 7) & mask;
+    while (t[h].used && t[h].key != key) h = (h + 1) & mask;
+    if (!t[h].used) {
+        t[h].used = 1;
+        t[h].key = key;
+    }
+    return &t[h];
+}
+
+double or_bench_slide(const or_tuple *R, const uint64_t *endR, const or_tuple *S, const uint64_t *endS,
+                      uint64_t n_events, uint64_t W, or_result *res) {
+    const uint64_t nR = n_events ? endR[n_events - 1] : 0, nS = n_events ? endS[n_events - 1] : 0;
+    uint64_t cap = 1024;
+    while (cap < 2 * (nR > nS ? nR : nS)) cap <<= 1;
+    slide_slot *tl = calloc(cap, sizeof(slide_slot)), *tr = calloc(cap, sizeof(slide_slot));
+    if (!tl || !tr) {
+        free(tl); free(tr);
+        return -1.0;
+    }
+    const uint64_t mask = cap - 1;
+    uint64_t count = 0, c1 = 0, pl = 0, pr = 0;  /* tuples of each side seen so far */
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (uint64_t e = 0; e < n_events; e++) {
+        const uint64_t el = endR[e], er = endS[e];
+        const uint64_t lo_old = pl > W ? pl - W : 0, lo_new = el > W ? el - W : 0;
+        const uint64_t ro_old = pr > W ? pr - W : 0, ro_new = er > W ? er - W : 0;
+        for (uint64_t i = lo_old; i < lo_new && i < pl; i++) { /* left tuples leaving */
+            slide_slot *o = slide_find(tr, mask, R[i].key);
+            const uint64_t a = or_hash_a(R[i].key, R[i].val);
+            count -= o->cnt;
+            c1 -= a * o->sum;
+            slide_slot *m = slide_find(tl, mask, R[i].key);
+            m->cnt--;
+            m->sum -= a;
+        }
+        for (uint64_t i = ro_old; i < ro_new && i < pr; i++) { /* right tuples leaving */
+            slide_slot *o = slide_find(tl, mask, S[i].key);
+            const uint64_t b = or_hash_b(S[i].key, S[i].val);
+            count -= o->cnt;
+            c1 -= o->sum * b;
+            slide_slot *m = slide_find(tr, mask, S[i].key);
+            m->cnt--;
+            m->sum -= b;
+        }
+        for (uint64_t i = pl > lo_new ? pl : lo_new; i < el; i++) { /* left tuples arriving */
+            slide_slot *o = slide_find(tr, mask, R[i].key);
+            const uint64_t a = or_hash_a(R[i].key, R[i].val);
+            count += o->cnt;
+            c1 += a * o->sum;
+            slide_slot *m = slide_find(tl, mask, R[i].key);
+            m->cnt++;
+            m->sum += a;
+        }
+        for (uint64_t i = pr > ro_new ? pr : ro_new; i < er; i++) { /* right tuples arriving */
+            slide_slot *o = slide_find(tl, mask, S[i].key);
+            const uint64_t b = or_hash_b(S[i].key, S[i].val);
+            count += o->cnt;
+            c1 += o->sum * b;
+            slide_slot *m = slide_find(tr, mask, S[i].key);
+            m->cnt++;
+            m->sum += b;
+        }
+        pl = el;
+        pr = er;
+        res[e].count = count;
+        res[e].c1 = c1;
+        res[e].c2 = 0;
+        res[e].pairs = 0;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(tl); free(tr);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

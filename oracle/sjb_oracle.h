@@ -87,6 +87,17 @@ int or_sliding_var(const or_tuple *R, const uint64_t *endR, const or_tuple *S, c
  * executable/runExperiment.cpp:163-166).  Returns elapsed seconds.        */
 double or_bench_lazy_hj(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uint64_t D,
                         int threads, or_result *total);
+/* Timed sliding-window baseline: after event i (left stream R up to
+ * endR[i], right stream S up to endS[i]) the window is the last W tuples of
+ * each side; res[i] = count and C1 of its join, maintained incrementally
+ * with one hash map per side (single thread).  Returns elapsed seconds.   */
+double or_bench_slide(const or_tuple *R, const uint64_t *endR, const or_tuple *S, const uint64_t *endS,
+                      uint64_t n_events, uint64_t W, or_result *res);
+/* The same windows with the INTER decomposition (runExperiment.cpp:167-170):
+ * `threads` joiners with one probe thread each take whole windows
+ * round-robin.  Returns elapsed seconds (-1 on allocation failure).       */
+double or_bench_lazy_hj_inter(const or_tuple *R, const or_tuple *S, uint64_t n, uint64_t W, uint64_t D,
+                              int threads, or_result *total);
 
 /* Learned partitioner root, float64: the 1-32-32-1 ReLU MLP
  *   y = w3 . relu(W2 relu(W1 x + b1) + b2) + b3,  loss = mean (y - t)^2

@@ -192,11 +192,13 @@ def test_driver_gpu_concurrent_sources(tmp_path, sjbmod, orc, prog, method):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prog,window_mb,data", [("LAZY", 1, 1 << 17), ("EAGER", 5, 1 << 19)])
+@pytest.mark.parametrize("prog,window_mb,data", [("LAZY", 1, 1 << 17), ("EAGER", 5, 2 * 5 * 65536)])
 def test_driver_gpu_bitmap_dense(tmp_path, sjbmod, orc, prog, window_mb, data):
     # 1000 distinct keys (the reference configs' value): a 1 MB lazy window
     # has ~4.2M (left tuple, 64-key word) masks with a match, a 5 MB eager
-    # window ~1.6M per batch pass; the mask buffer grows to what is needed
+    # window ~1.6M per batch pass; the mask buffer grows to what is needed.
+    # Streams hold whole windows: eager executors also join a partial last
+    # window, which expected_lazy does not count.
     print(f"case driver bitmap D=1000 {prog} {window_mb}MB", flush=True)
     rows = run_driver(tmp_path, yaml_cfg("NLJ", "GPU", prog, "Bitmap", window_mb=window_mb, data=data, keys=1000),
                       env={"SJB_HAVE_GPU": "1"})
