@@ -123,6 +123,7 @@ SIGNATURES = {
     "sjb_lazy_execute": (_i, [_p, _i, _i, _u32, _p, _p, _u64, _p, _p, _u64, _u64, _p, _u64,
                               ctypes.POINTER(Stats)]),
     "sjb_lazy_join_async": (_i, [_p, _i, _u32, _p, _u64, _p, _u64, _p]),
+    "sjb_lazy_join_noalloc": (_i, [_i, _u32, _p, _u64, _p, _u64, _p, _p]),
     "sjb_lazy_join_batch": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _p]),
     "sjb_batch_plan_create": (_i, [_p, _i, _u32, _p, _p, _p, _p, _u32, _p, ctypes.POINTER(_p)]),
     "sjb_batch_plan_launch": (_i, [_p]),

@@ -402,6 +402,25 @@ int sjb_pk_stop(sjb_ctx_t *ctx);
 /* Number of persistent-kernel launches so far (relaunches after idle exits). */
 uint64_t sjb_pk_launches(sjb_ctx_t *ctx);
 
+/* ---- allocation-free window join ---------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one HJ or SMJ window like sjb_lazy_join_async
+ * (count + C1, with SJB_F_C2 also C2, or SJB_F_GROUPED), without a context,
+ * enqueued on the caller's `stream` (a hipStream_t of the current device;
+ * null = that device's null stream); the result lands in device memory
+ * d_result when the stream reaches it.  It calls no hipMalloc / hipFree and
+ * clears its counters with kernels (no memset nodes).  Scratch memory is one
+ * static device array of about 19 MB per GPU, held until the process ends,
+ * so each side holds at most SJB_NOALLOC_MAX_TUPLES tuples (SJB_E_CAPACITY
+ * beyond); other flags and NLJ return SJB_E_UNSUPPORTED.  Every call on a
+ * GPU shares that GPU's array: calls that overlap in time on one GPU race,
+ * so order them (one stream, or events).                                  */
+#define SJB_NOALLOC_MAX_TUPLES (1u << 20)
+int sjb_lazy_join_noalloc(int algorithm, uint32_t flags, const sjb_tuple_t *d_left, uint64_t n_left,
+                          const sjb_tuple_t *d_right, uint64_t n_right, sjb_window_result_t *d_result,
+                          void *stream);
+
 /* ---- data generation (host, deterministic) ----------------------------- */
 /* Fills n tuples: key distribution by `dist` (0 uniform over [0,key_range),
  * 1 Zipf(theta) over [0,key_range) with ranks scrambled by a fixed
