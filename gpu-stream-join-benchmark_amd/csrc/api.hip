@@ -466,6 +466,28 @@ int sjb_sync(sjb_ctx_t *c) {
 
 uint64_t sjb_device_bytes(sjb_ctx_t *c) { return c ? c->bytes : 0; }
 
+int sjb_host_alloc(uint64_t bytes, void **p) {
+    if (!p) return SJB_E_INVALID;
+    *p = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return SJB_E_NODEVICE;
+    }
+    if (hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return SJB_E_NOMEM;
+    }
+    return SJB_OK;
+}
+
+int sjb_host_free(void *p) {
+    if (!p) return SJB_OK;
+    SJB_HIP_TRY(hipHostFree(p));
+    return SJB_OK;
+}
+
 int sjb_lazy_join(sjb_ctx_t *c, int algorithm, int method, uint32_t flags, const sjb_tuple_t *d_left,
                   uint64_t n_left, const sjb_tuple_t *d_right, uint64_t n_right,
                   sjb_result_tuple_t *out, uint64_t out_cap, sjb_stats_t *stats) {

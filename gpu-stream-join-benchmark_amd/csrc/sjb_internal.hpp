@@ -246,12 +246,12 @@ struct sjb_ctx {
 
     // persistent-kernel eager executor (pk.hip)
     hipStream_t pk_stream = nullptr;   // runs the persistent kernel
-    hipStream_t pk_copy = nullptr;     // batch copies + doorbell writes
-    void *d_pk_ctl = nullptr;          // doorbell / control words (device)
-    void *d_pk_descs = nullptr;        // work descriptors (device ring)
-    sjb_tuple_t *d_pk_slots = nullptr; // batch slots (device ring)
+    hipStream_t pk_copy = nullptr;     // control-word resets before a relaunch
+    void *d_pk_ctl = nullptr;          // forwarded posts, completion, claims (device)
+    void *h_pk_host = nullptr;         // doorbell + descriptor ring (coherent pinned host)
+    sjb_tuple_t *h_pk_slots = nullptr; // pinned copies of pageable batches (ring)
     void *h_pk_res = nullptr;          // results + completion (coherent pinned host)
-    void *h_pk_stage = nullptr;        // pinned staging of descriptors
+    uint64_t *h_pk_word = nullptr;     // pinned source of control-word resets
     uint64_t pk_seq = 0;               // next sequence number
     uint64_t pk_slot_cap = 0;          // tuples per batch slot
     uint64_t pk_n[2] = {0, 0};         // host view of the state sizes

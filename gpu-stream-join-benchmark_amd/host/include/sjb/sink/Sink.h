@@ -4,7 +4,8 @@
 // members sinkBuffer (a ring of sinkBufferSize result tuples), counterGPU
 // (one counter that the Atomic executors add to; callers reset it with
 // counterGPU[0] = 0, as src/experiment/Runner.cpp:44 does).  Both are
-// ordinary host memory here: the C ABI stages device output into them.
+// page-locked host memory when a device is present (else ordinary memory,
+// which the C ABI stages device output into).
 // addToStatistics (a no-op in the reference, src/sjb/sink/Sink.cpp:65-67)
 // accumulates the per-window checksums that ExecutionStatistic carries.
 #pragma once
@@ -70,6 +71,7 @@ private:
     std::atomic<uint64_t> totalBuildTime{0}, totalProbeTime{0}, totalSortTime{0}, totalMergeTime{0};
     std::atomic<uint64_t> totalExecutedWindows{0};
     std::atomic<uint64_t> checksum1{0}, checksum2{0};
+    bool bufferPinned = false, counterPinned = false;
     std::chrono::time_point<std::chrono::high_resolution_clock> processingStartTime;
     std::mutex progressMutex;
     std::vector<Progress> progress;

@@ -41,8 +41,16 @@ public:
     void onEndOfStream() override {}
     // Joins every window whose batches have all arrived (in window order).
     void execute();
+    ~CountBased() override;
+    CountBased(const CountBased &) = delete;
+    CountBased &operator=(const CountBased &) = delete;
 
-    std::vector<Tuple> leftRingBuffer, rightRingBuffer;
+    // ring buffers of nSlots * batchSize tuples per side, page-locked when a
+    // device is present (include/sjb/windowing/lazy/CountBased.cuh:35-36,
+    // :178-179), so the GPU executors copy windows from them asynchronously
+    Tuple *leftRingBuffer{};
+    Tuple *rightRingBuffer{};
+    bool leftPinned = false, rightPinned = false;
 
 private:
     struct Side {

@@ -4,6 +4,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include "sjb/utils/Pinned.hpp"
 #include "sjb/windowing/eager/CountBased.hpp"
 #include "sjb/windowing/lazy/CountBased.hpp"
 
@@ -19,10 +20,21 @@ CountBased::CountBased(Sink &sink, uint64_t ringBufferSize, uint64_t ringBufferM
     // The ring must hold at least one window per side plus a batch in flight.
     if (this->ringBufferSize < windowSize + batchSize) this->ringBufferSize = windowSize + batchSize;
     nSlots = this->ringBufferSize / batchSize;
-    leftRingBuffer.resize(nSlots * batchSize);
-    rightRingBuffer.resize(nSlots * batchSize);
+    const uint64_t bytes = nSlots * batchSize * sizeof(Tuple);
+    leftRingBuffer = static_cast<Tuple *>(sjb_host::pinnedAlloc(bytes, &leftPinned));
+    try {
+        rightRingBuffer = static_cast<Tuple *>(sjb_host::pinnedAlloc(bytes, &rightPinned));
+    } catch (...) {
+        sjb_host::pinnedFree(leftRingBuffer, leftPinned);
+        throw;
+    }
     for (auto &sd : sides) sd.released.assign(nSlots, 0);
     for (auto &j : js) joiners.push_back(j);
+}
+
+CountBased::~CountBased() {
+    sjb_host::pinnedFree(leftRingBuffer, leftPinned);
+    sjb_host::pinnedFree(rightRingBuffer, rightPinned);
 }
 
 void CountBased::onIncoming(Tuple *tupleBuffer, bool left, uint64_t i) {
@@ -39,7 +51,7 @@ void CountBased::onIncoming(Tuple *tupleBuffer, bool left, uint64_t i) {
             slotCv.wait_until(g, std::chrono::system_clock::now() + std::chrono::microseconds(1));
         }
     }
-    std::memcpy((left ? leftRingBuffer : rightRingBuffer).data() + slot * batchSize, tupleBuffer,
+    std::memcpy((left ? leftRingBuffer : rightRingBuffer) + slot * batchSize, tupleBuffer,
                 batchSize * sizeof(Tuple));
     {
         std::lock_guard<std::mutex> g(mtx);
@@ -95,7 +107,7 @@ void CountBased::execute() {
         }
         ExecutionStatistic es;
         try {
-            es = joiner->execute(leftRingBuffer.data(), rightRingBuffer.data(), li, ri, sink);
+            es = joiner->execute(leftRingBuffer, rightRingBuffer, li, ri, sink);
         } catch (...) {
             std::lock_guard<std::mutex> g(joinerMtx);
             joiners.push_back(joiner);

@@ -12,6 +12,8 @@
 
 #include "sjb/sink/Sink.h"
 #include "sjb/source/Source.hpp"
+#include "sjb/utils/Pinned.hpp"
+#include "sjb/utils/PowerMeasurementAgent.hpp"
 #include "sjb/windowing/WindowingFactory.hpp"
 
 std::string Runner::csvHeader(bool extended) {
@@ -25,7 +27,9 @@ std::string Runner::csvHeader(bool extended) {
 // Runner.cpp:14-104
 Runner::RunResult Runner::runExperimentWithConfig(ExperimentConfig config, bool trackProgress) {
     const DataConfig &d = config.dataConfig;
-    std::vector<Tuple> left(d.dataSize), right(d.dataSize);
+    // page-locked source streams (Runner.cpp:18-33), so GPU executors copy
+    // batches and windows from them asynchronously
+    sjb_host::PinnedArray<Tuple> left(d.dataSize), right(d.dataSize);
     std::thread gl([&] {
         DataGenerator::generate(left.data(), d.dataSize, d.zipfTheta, d.tsMode, d.numDistinctKey, 123,
                                 d.timestampUpperBound);
@@ -39,10 +43,13 @@ Runner::RunResult Runner::runExperimentWithConfig(ExperimentConfig config, bool 
     sink.counterGPU[0] = 0;  // Runner.cpp:44
     auto windowing = Windowing::WindowingFactory::create(config.algorithmConfig, config.dataConfig,
                                                          config.queryConfig, config.engineConfig, sink);
+    PowerMeasurementAgent pma;  // Runner.cpp:36-62: polls while the source runs
+    if (config.measurePower) pma.start();
     const auto t0 = std::chrono::high_resolution_clock::now();
     sink.markProcessingStart();
     source.start(left.data(), right.data(), config.engineConfig.nSourceThreads, windowing);
     const auto t1 = std::chrono::high_resolution_clock::now();
+    if (config.measurePower) pma.end();
 
     RunResult r;
     r.msElapsed = uint64_t(std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count());

@@ -6,19 +6,26 @@
 #include <cstring>
 #include <new>
 
+#include "sjb/utils/Pinned.hpp"
+
+// sinkBuffer and counterGPU are page-locked when a device is present
+// (Sink.cpp:8-9 allocates both with cudaMallocHost), so device kernels
+// write pair rows straight into the ring.
 Sink::Sink(uint64_t sinkBufferSize) : sinkBufferSize(sinkBufferSize) {
-    sinkBuffer = static_cast<ResultTuple *>(std::calloc(sinkBufferSize ? sinkBufferSize : 1, sizeof(ResultTuple)));
-    counterGPU = static_cast<unsigned long long *>(std::calloc(1, sizeof(unsigned long long)));
-    if (!sinkBuffer || !counterGPU) {
-        std::free(sinkBuffer);
-        std::free(counterGPU);
-        throw std::bad_alloc();
+    sinkBuffer = static_cast<ResultTuple *>(
+        sjb_host::pinnedAlloc((sinkBufferSize ? sinkBufferSize : 1) * sizeof(ResultTuple), &bufferPinned));
+    try {
+        counterGPU = static_cast<unsigned long long *>(
+            sjb_host::pinnedAlloc(sizeof(unsigned long long), &counterPinned));
+    } catch (...) {
+        sjb_host::pinnedFree(sinkBuffer, bufferPinned);
+        throw;
     }
 }
 
 Sink::~Sink() {
-    std::free(sinkBuffer);
-    std::free(counterGPU);
+    sjb_host::pinnedFree(sinkBuffer, bufferPinned);
+    sjb_host::pinnedFree(counterGPU, counterPinned);
 }
 
 void Sink::addToCounter(uint64_t currentMatches, bool isWindowClosed) {

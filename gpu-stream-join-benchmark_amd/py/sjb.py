@@ -119,6 +119,8 @@ SIGNATURES = {
     "sjb_stream": (_p, [_p]),
     "sjb_sync": (_i, [_p]),
     "sjb_device_bytes": (_u64, [_p]),
+    "sjb_host_alloc": (_i, [_u64, ctypes.POINTER(_p)]),
+    "sjb_host_free": (_i, [_p]),
     "sjb_lazy_join": (_i, [_p, _i, _i, _u32, _p, _u64, _p, _u64, _p, _u64, ctypes.POINTER(Stats)]),
     "sjb_lazy_execute": (_i, [_p, _i, _i, _u32, _p, _p, _u64, _p, _p, _u64, _u64, _p, _u64,
                               ctypes.POINTER(Stats)]),

@@ -151,6 +151,14 @@ int sjb_sync(sjb_ctx_t *ctx);
 /* Device bytes currently held by the ctx.                                  */
 uint64_t sjb_device_bytes(sjb_ctx_t *ctx);
 
+/* Page-locked host memory the device copies from (and writes to)
+ * asynchronously; SJB_E_NODEVICE without a device.  Replaces the
+ * cudaMallocHost calls of the reference's source streams, ring buffers and
+ * sink (src/experiment/Runner.cpp:23,29,
+ * include/sjb/windowing/lazy/CountBased.cuh:35-36, src/sjb/sink/Sink.cpp:8-9). */
+int sjb_host_alloc(uint64_t bytes, void **p);
+int sjb_host_free(void *p);
+
 /* ---- lazy (tumbling) window join --------------------------------------- */
 /* One window, inputs already resident in device memory.  Replaces
  * BaseLazyExecutor::execute (include/sjb/windowing/lazy/executor/BaseLazyExecutor.hpp:26-30)

@@ -114,3 +114,13 @@ def test_generator_scramble_is_bijective(sjbmod):
     # must appear (a non-bijective scramble would leave holes)
     a = sjbmod.generate(1 << 14, sjbmod.DIST_ZIPF, 100, 1e-9, seed=4)
     assert len(np.unique(a["key"])) == 100
+
+
+def test_host_alloc_without_device(sjbmod):
+    if _gpu_available():
+        pytest.skip("a device is present: the GPU tests use sjb_host_alloc")
+    L = sjbmod.lib()
+    p = ctypes.c_void_p(1)
+    assert L.sjb_host_alloc(4096, ctypes.byref(p)) == -6 and not p.value
+    assert L.sjb_host_free(None) == 0
+    assert L.sjb_host_alloc(16, None) == -1

@@ -301,3 +301,43 @@ def test_overlap_async_equals_plain(sjbmod, torch_mod, orc, alg):
     for w in (0, nw - 1):
         o, _ = orc.lazy_hj(R[w * W:(w + 1) * W], S[w * W:(w + 1) * W], 1 << 12)
         assert (int(outs[1][w]["count"]), int(outs[1][w]["checksum1"]), int(outs[1][w]["checksum2"])) == o.triple()
+
+
+def test_pinned_ring_slots_and_output(ctx, sjbmod, orc):
+    """Ring slots and pair rows in page-locked memory from sjb_host_alloc
+    (asynchronous copies in, device writes straight into the host rows)."""
+    import ctypes
+    L = sjbmod.lib()
+    B, nslots = 1024, 16
+    ringL, ringR, outp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.sjb_host_alloc(nslots * B * 16, ctypes.byref(ringL)) == 0
+    assert L.sjb_host_alloc(nslots * B * 16, ctypes.byref(ringR)) == 0
+    cap = 1 << 16
+    assert L.sjb_host_alloc(cap * 32, ctypes.byref(outp)) == 0
+    try:
+        rl = np.ctypeslib.as_array((ctypes.c_uint8 * (nslots * B * 16)).from_address(ringL.value)).view(sjbmod.TUPLE_DTYPE)
+        rr = np.ctypeslib.as_array((ctypes.c_uint8 * (nslots * B * 16)).from_address(ringR.value)).view(sjbmod.TUPLE_DTYPE)
+        rl[:] = make(nslots * B, 700, 201)
+        rr[:] = make(nslots * B, 700, 202)
+        li = np.array([3, 4, 5, 9, 0], dtype=np.uint64) * B
+        ri = np.array([15, 1, 2, 7, 8], dtype=np.uint64) * B
+        Lw = np.concatenate([rl[i:i + B] for i in li])
+        Rw = np.concatenate([rr[i:i + B] for i in ri])
+        o, _ = orc.lazy_hj(Lw, Rw, 1000)
+        for alg in (HJ, SMJ, NLJ):
+            rc, st = ctx.lazy_execute(alg, NO, F_C2, ringL.value, li, ringR.value, ri, B)
+            assert rc == 0 and (st.result_count, st.checksum1, st.checksum2) == o.triple()
+        st = ctx.lazy_join(HJ, AT, 0, to_dev_np(Lw), len(Lw), to_dev_np(Rw), len(Rw), outp.value, cap)
+        rows = np.ctypeslib.as_array((ctypes.c_uint8 * (cap * 32)).from_address(outp.value))
+        rows = rows.view(np.dtype([("lkey", "<u4"), ("rkey", "<u4"), ("lval", "<u4"), ("rval", "<u4"),
+                                   ("lts", "<u8"), ("rts", "<u8")]))[: st.pairs_written]
+        assert st.pairs_written == o.count
+        assert np.array_equal(sort_rows(rows.copy()), sort_rows(np_pair_rows(Lw, Rw)))
+    finally:
+        for p in (ringL, ringR, outp):
+            assert L.sjb_host_free(p) == 0
+
+
+def to_dev_np(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to("cuda")

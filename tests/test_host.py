@@ -23,7 +23,7 @@ def run_driver(tmp_path, yaml_text, name="out.csv", env=None):
     e = dict(os.environ)
     e.update(env or {})
     p = subprocess.run([os.path.join(BIN, "runExperiment"), str(cfg), str(out)], capture_output=True,
-                       text=True, timeout=600, env=e)
+                       text=True, timeout=600, env=e, cwd=str(tmp_path))
     assert p.returncode == 0, p.stderr[-2000:]
     with open(out) as f:
         return list(csv.DictReader(f))
@@ -129,6 +129,18 @@ def test_driver_cpu_concurrent_sources(tmp_path, sjbmod, orc, prog):
         assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1, par
 
 
+def test_driver_power_without_gpu(tmp_path, sjbmod, orc):
+    # measurePowerUsage: "Yes" with no GPU / no AMD SMI reading: the run
+    # goes on and reports the same results, without power samples
+    from conftest import _gpu_available
+    if _gpu_available():
+        pytest.skip("a GPU is present: test_driver_gpu_power covers the polling")
+    y = yaml_cfg("HJ", "CPU", "LAZY", "NoOutput").replace('measurePowerUsage: "No"', 'measurePowerUsage: "Yes"')
+    rows = run_driver(tmp_path, y)
+    cnt, c1 = expected_lazy(sjbmod, orc, 1 << 17, 65536, 1000)
+    assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
+
+
 def test_driver_cpu_eager(tmp_path, sjbmod, orc):
     data = 1 << 17
     rows = run_driver(tmp_path, yaml_cfg("HJ", "CPU", "EAGER", "NoOutput", data=data))
@@ -204,3 +216,21 @@ def test_driver_gpu_bitmap_dense(tmp_path, sjbmod, orc, prog, window_mb, data):
                       env={"SJB_HAVE_GPU": "1"})
     cnt, c1 = expected_lazy(sjbmod, orc, data, window_mb * 65536, 1000)
     assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
+
+
+@pytest.mark.gpu
+def test_driver_gpu_power(tmp_path, sjbmod, orc):
+    # AMD SMI polling while the source runs: power_data.txt in the working
+    # directory, "<ms since epoch>,<watts>" per line (the reference format)
+    print("case driver power", flush=True)
+    data = 1 << 21
+    y = yaml_cfg("HJ", "GPU", "LAZY", "NoOutput", data=data, keys=5000).replace(
+        'measurePowerUsage: "No"', 'measurePowerUsage: "Yes"')
+    rows = run_driver(tmp_path, y, env={"SJB_HAVE_GPU": "1"})
+    cnt, c1 = expected_lazy(sjbmod, orc, data, 65536, 5000)
+    assert int(rows[0]["match_count"]) == cnt and int(rows[0]["checksum1"]) == c1
+    lines = (tmp_path / "power_data.txt").read_text().split()
+    assert len(lines) >= 1
+    ts, w = zip(*[(int(a), float(b)) for a, b in (ln.split(",") for ln in lines)])
+    assert all(10.0 <= x <= 3000.0 for x in w), w[:10]
+    assert list(ts) == sorted(ts)
