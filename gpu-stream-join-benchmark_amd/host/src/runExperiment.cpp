@@ -141,9 +141,15 @@ int main(int argc, char **argv) {
                                                 c.algorithmConfig.cpuhjConfig.nProheThreads = std::max<uint64_t>(1, hw / 4);
                                                 c.algorithmConfig.cpusmjConfig.nSortThreads = std::max<uint64_t>(1, hw / 4);
                                             }
-                                            // one device, one executor per joiner thread:
-                                            // GPU windows are joined by one executor
-                                            if (dev == Windowing::GPU) c.engineConfig.joinerThreads = 1;
+                                            // GPU joiners each own a device context and
+                                            // stream, so windows joined by different
+                                            // joiners overlap (one window's host-to-device
+                                            // copy with another's join); past 4 contexts
+                                            // they only add device memory (the copy
+                                            // engines and the device are shared)
+                                            if (dev == Windowing::GPU)
+                                                c.engineConfig.joinerThreads =
+                                                    std::min<uint64_t>(c.engineConfig.joinerThreads, 4);
                                             configs.push_back(c);
                                         }
         Runner runner;
