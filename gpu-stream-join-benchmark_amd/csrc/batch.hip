@@ -42,6 +42,7 @@ struct sjb_batch_plan {
     bool peel = false;
     bool drift_fallback = false;      // learned CDF judged stale on this batch, radix ranges used
     bool drift_refit = false;         // ... and refitted to this batch's key sample instead
+    bool peel_sampled = false;        // SJB_F_PEEL: heavy keys of this batch's sample, no model
     sjb::RmiParams rmi_refit{};       // the refitted model (host copy)
     sjb::RmiParams *d_rmi = nullptr;  // ... and on the device
     size_t cap_rmi = 0;
@@ -121,6 +122,22 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     // not beat radix — the batch uses radix ranges.  Peeled heavy keys stay.
     p->drift_fallback = false;
     p->drift_refit = false;
+    p->peel_sampled = false;
+    if (p->kind != PART_LEARNED && (p->flags & SJB_F_PEEL) && (p->flags & SJB_F_GROUPED) && nl_tot > 0) {
+        // model-free peeling: heavy keys of this batch's own key sample,
+        // radix ranges (SMJ) or hash partitions (HJ) for the rest
+        uint32_t nh = 0;
+        int rc = sampled_heavy(c, L + loff[0], nl_tot, &p->rmi_refit, &nh);
+        if (rc) return rc;
+        if (nh > 0) {
+            if ((rc = grow(p, &p->d_rmi, &p->cap_rmi, 1))) return rc;
+            SJB_HIP_TRY(hipMemcpyAsync(p->d_rmi, &p->rmi_refit, sizeof(RmiParams), hipMemcpyHostToDevice,
+                                       c->stream));
+            p->peel = true;
+            p->peel_sampled = true;
+            p->n_heavy = nh;
+        }
+    }
     if (p->kind == PART_LEARNED && nl_tot > 0) {
         std::vector<uint32_t> keys;
         int rc = rmi_sample_keys(c, L + loff[0], nl_tot, keys);
@@ -269,7 +286,7 @@ static int plan_build(sjb_batch_plan *p, const sjb_tuple_t *L, const uint64_t *l
     a.expand = uint32_t(p->tune.expand);
     const bool grouped = (p->flags & SJB_F_GROUPED) != 0;
     a.idcap = uint32_t(join_idcap(p->alg, grouped));
-    a.rmi = p->drift_refit ? p->d_rmi : c->d_rmi;
+    a.rmi = (p->drift_refit || p->peel_sampled) ? p->d_rmi : c->d_rmi;
     a.heavy = p->peel ? p->d_heavy : nullptr;
     p->dyn_part = partition_lds_bytes_chunk(max_logF, int(chunk));
     p->dyn_join = join_view_bytes(p->alg, grouped, max_ncL, max_ncR);
@@ -434,7 +451,7 @@ int sjb_batch_plan_info(sjb_batch_plan_t *p, sjb_batch_info_t *info) {
     info->part_kind = uint32_t(p->kind);
     info->drift_fallback = p->drift_fallback ? 1u : 0u;
     info->drift_refit = p->drift_refit ? 1u : 0u;
-    info->reserved = 0;
+    info->n_heavy = p->peel ? p->n_heavy : 0u;
     info->tune.fanout_bias = p->tune.fanout_bias;
     info->tune.chunk_log2 = uint32_t(p->tune.chunk_log2);
     info->tune.blocks_per_cu = uint32_t(p->tune.blocks_per_cu);

@@ -289,19 +289,22 @@ __global__ void k_leaf_fit(const double *__restrict__ st, RmiParams *__restrict_
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Each buffer is allocated on its own first use: the key sample alone is
+// also needed without a fit (sampled heavy keys, drift judgement).
 static int fit_buffers(sjb_ctx *c, uint32_t steps) {
     int rc;
-    if (!c->d_fit_keys) {
-        if ((rc = dev_alloc(c, (void **)&c->d_fit_keys, kFitSample * sizeof(uint32_t))) ||
-            (rc = dev_alloc(c, (void **)&c->d_fit_x, kFitMax * sizeof(float))) ||
-            (rc = dev_alloc(c, (void **)&c->d_fit_t, kFitMax * sizeof(float))) ||
-            (rc = dev_alloc(c, (void **)&c->d_mlp, 4 * kMlpParams * sizeof(float))) ||
-            (rc = dev_alloc(c, (void **)&c->d_fit_acc, (1 + 5 * kRmiLeaves) * sizeof(double))))
-            return rc;
-    }
+    if ((!c->d_fit_keys && (rc = dev_alloc(c, (void **)&c->d_fit_keys, kFitSample * sizeof(uint32_t)))) ||
+        (!c->d_fit_x && (rc = dev_alloc(c, (void **)&c->d_fit_x, kFitMax * sizeof(float)))) ||
+        (!c->d_fit_t && (rc = dev_alloc(c, (void **)&c->d_fit_t, kFitMax * sizeof(float)))) ||
+        (!c->d_mlp && (rc = dev_alloc(c, (void **)&c->d_mlp, 4 * kMlpParams * sizeof(float)))) ||
+        (!c->d_fit_acc && (rc = dev_alloc(c, (void **)&c->d_fit_acc, (1 + 5 * kRmiLeaves) * sizeof(double)))))
+        return rc;
     if (c->loss_hist_cap < steps + 1) {
-        if ((rc = dev_alloc(c, (void **)&c->d_loss_hist, (steps + 1) * sizeof(float)))) return rc;
-        c->loss_hist_cap = steps + 1;
+        void *p = c->d_loss_hist;
+        rc = dev_regrow(c, &p, c->d_loss_hist ? c->loss_hist_cap * sizeof(float) : 0, (steps + 1) * sizeof(float));
+        c->d_loss_hist = static_cast<float *>(p);
+        c->loss_hist_cap = c->d_loss_hist ? steps + 1 : 0;
+        if (rc) return rc;
     }
     return SJB_OK;
 }
@@ -426,6 +429,40 @@ int rmi_sample_keys(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, std::vector<ui
     keys.resize(ns);
     SJB_HIP_TRY(hipMemcpyAsync(keys.data(), c->d_fit_keys, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
     SJB_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SJB_OK;
+}
+
+// Heavy keys of a batch without a model (the radix baseline with peeling):
+// the keys occurring at least kPeelMin times in an evenly spaced sample of
+// up to kFitSample keys of T[0, n), the kMaxHeavy most frequent first, with
+// their lookup table; no CDF (out->root_kind etc. unused).  Returns the
+// number of heavy keys in *n_heavy.
+constexpr uint32_t kPeelMin = 3;
+int sampled_heavy(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, RmiParams *out, uint32_t *n_heavy) {
+    std::memset(out, 0, sizeof(*out));
+    *n_heavy = 0;
+    if (n == 0) return SJB_OK;
+    if (!c->d_fit_keys) {
+        int rc = dev_alloc(c, (void **)&c->d_fit_keys, kFitSample * sizeof(uint32_t));
+        if (rc) return rc;
+    }
+    std::vector<uint32_t> keys;
+    int rc = rmi_sample_keys(c, T, n, keys);
+    if (rc) return rc;
+    std::sort(keys.begin(), keys.end());
+    std::vector<std::pair<uint32_t, uint32_t>> runs;  // (count, key)
+    for (size_t i = 0; i < keys.size();) {
+        size_t j = i;
+        while (j < keys.size() && keys[j] == keys[i]) j++;
+        if (j - i >= kPeelMin) runs.emplace_back(uint32_t(j - i), keys[i]);
+        i = j;
+    }
+    std::stable_sort(runs.begin(), runs.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    if (runs.size() > size_t(kMaxHeavy)) runs.resize(kMaxHeavy);
+    out->n_heavy = uint32_t(runs.size());
+    for (uint32_t h = 0; h < out->n_heavy; h++) out->heavy[h] = runs[h].second;
+    heavy_table_build(*out);
+    *n_heavy = out->n_heavy;
     return SJB_OK;
 }
 

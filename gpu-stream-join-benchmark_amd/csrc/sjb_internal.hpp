@@ -327,6 +327,7 @@ constexpr int kDriftNone = 0, kDriftRefit = 1, kDriftRanges = 2;
 int rmi_judge(const sjb_ctx *c, const std::vector<uint32_t> &keys, bool peel, bool allow_refit,
               RmiParams *out);
 int rmi_sample_keys(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, std::vector<uint32_t> &keys);
+int sampled_heavy(sjb_ctx *c, const sjb_tuple_t *T, uint64_t n, RmiParams *out, uint32_t *n_heavy);
 int join_idcap(int alg, bool grouped);
 
 // ---- pointer checks / output staging (api.hip) ----

@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("SJB_LIB") or os.path.join(PKG, "lib", "libsjb_hip.so"
 # enums (include/sjb/windowing/BaseWindowing.cuh:9-40 values)
 ALG = {"EMPTY": 0, "NLJ": 1, "HJ": 2, "SMJ": 3}
 OUT = {"NoOutput": 0, "CountKernel": 1, "EstimatedSelectivity": 2, "Bitmap": 3, "Atomic": 4}
-F_C2, F_GROUPED, F_LEARNED, F_TIMING, F_OVERLAP, F_NO_REFIT = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
+F_C2, F_GROUPED, F_LEARNED, F_TIMING, F_OVERLAP, F_NO_REFIT, F_PEEL = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20, 0x40
 DIST_UNIFORM, DIST_ZIPF = 0, 1
 
 E = {
@@ -68,7 +68,7 @@ class BatchInfo(ctypes.Structure):
                 ("n_items", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("tune", BatchTune),
                 ("part_kind", ctypes.c_uint32), ("drift_fallback", ctypes.c_uint32),
-                ("drift_refit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("drift_refit", ctypes.c_uint32), ("n_heavy", ctypes.c_uint32)]
 
 
 class RmiConfig(ctypes.Structure):

@@ -77,6 +77,13 @@ extern "C" {
 #define SJB_F_NO_REFIT 0x20u   /* batched + SJB_F_LEARNED: a model found stale */
                                /* on the batch's keys is not refitted to them; */
                                /* the batch uses radix ranges instead          */
+#define SJB_F_PEEL 0x40u       /* batched + SJB_F_GROUPED, no SJB_F_LEARNED:   */
+                               /* the most frequent keys of a sample of the    */
+                               /* batch's left stream (>= 3 of 16384 samples,  */
+                               /* at most 32) are aggregated apart; the rest is */
+                               /* hash (HJ) / radix-range (SMJ) partitioned    */
+                               /* as without it — the model-free baseline of   */
+                               /* the learned partitioner                      */
 
 /* ---- data layout ------------------------------------------------------- */
 /* Identical to Tuple (include/sjb/utils/Tuple.hpp:6-11): 16 B, AoS.        */
@@ -256,7 +263,7 @@ typedef struct sjb_batch_info {
                             /* sample (root = the sample's CDF as a table,   */
                             /* leaves = chords of it; heavy keys kept) and   */
                             /* partitions by it                              */
-    uint32_t reserved;
+    uint32_t n_heavy;       /* keys aggregated apart (learned or SJB_F_PEEL) */
 } sjb_batch_info_t;
 
 int sjb_batch_plan_create(sjb_ctx_t *ctx, int algorithm, uint32_t flags,

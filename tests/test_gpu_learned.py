@@ -9,9 +9,11 @@ root trained on the device).
   count and C1 with peeled heavy keys) against the oracle: the model only
   moves balance, never results.
 * What training gains: on the next window of the drifting mixture, the
-  trained MLP root gives lower partition imbalance and fewer fix-ups
-  (tuples in partitions larger than one LDS tile) than the linear-root RMI
-  and than radix (range) partitioning on the key bits.
+  trained MLP root gives lower partition imbalance, fewer fix-ups (tuples
+  in partitions larger than one LDS tile) and a faster SMJ join than the
+  linear-root RMI, which beats radix (range) partitioning on the key bits.
+* The model-free baseline (SJB_F_PEEL: heavy keys of the batch's own key
+  sample aggregated apart) gives the plain grouped results.
 * Drift: a batch plan judges the model on a sample of its own keys and
   refits a stale one to that sample, or falls back to radix ranges with
   SJB_F_NO_REFIT; neither changes the results.
@@ -122,8 +124,28 @@ def test_learned_batch_equals_single(fitted, torch_mod, sjbmod):
             assert (int(res[w]["count"]), int(res[w]["checksum1"])) == (st.result_count, st.checksum1)
 
 
+def join_ms(c, torch_mod, alg, flags, dR, dS, n, reps=5):
+    """Median device time (ms) of one grouped window join, and its result."""
+    stream = torch_mod.cuda.ExternalStream(c.stream)
+    d_res = torch_mod.zeros(32, dtype=torch_mod.uint8, device="cuda")
+    c.lazy_join_async(alg, flags, dR, n, dS, n, d_res.data_ptr())
+    c.sync()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch_mod.cuda.Event(enable_timing=True), torch_mod.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        c.lazy_join_async(alg, flags, dR, n, dS, n, d_res.data_ptr())
+        e1.record(stream)
+        c.sync()
+        torch_mod.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    r = d_res.cpu().numpy().view(np.uint64)
+    return float(np.median(ms)), (int(r[0]), int(r[1]))
+
+
 def test_trained_network_gains(sjbmod, torch_mod):
-    # the next window after the training windows: MLP root vs linear root vs radix
+    # the next window after the training windows: MLP root vs linear root vs
+    # radix, on partition balance and on the SMJ join's device time
     train = training_windows(sjbmod, 1 << 20, 0, 4, seed=70)
     d_train = to_dev(torch_mod, train)
     n = 1 << 22
@@ -132,18 +154,48 @@ def test_trained_network_gains(sjbmod, torch_mod):
     dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
     with sjbmod.Context(max_window=n, distinct_keys=1 << 16) as c:
         radix = c.partition_report(SMJ, F_GROUPED, dR, n, dS, n)
+        t_radix, res_radix = join_ms(c, torch_mod, SMJ, F_GROUPED, dR, dS, n)
         lin_fit = c.rmi_fit(d_train, len(train), root=sjbmod.RMI_LINEAR)
         lin = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
+        t_lin, res_lin = join_ms(c, torch_mod, SMJ, F_GROUPED | F_LEARNED, dR, dS, n)
         mlp_fit = c.rmi_fit(d_train, len(train), root=sjbmod.RMI_MLP, steps=400, lr=0.01)
         mlp = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
+        t_mlp, res_mlp = join_ms(c, torch_mod, SMJ, F_GROUPED | F_LEARNED, dR, dS, n)
         c.rmi_clear()
         plain = c.partition_report(SMJ, F_GROUPED | F_LEARNED, dR, n, dS, n)
-    print("radix", radix.as_dict(), "\nlinear", lin.as_dict(), lin_fit.as_dict(),
-          "\nmlp", mlp.as_dict(), mlp_fit.as_dict(), flush=True)
+    print("radix", radix.as_dict(), t_radix, "ms\nlinear", lin.as_dict(), lin_fit.as_dict(), t_lin,
+          "ms\nmlp", mlp.as_dict(), mlp_fit.as_dict(), t_mlp, "ms", flush=True)
     assert radix.kind == 1 and lin.kind == 2 and mlp.kind == 2 and plain.kind == 1
     assert mlp_fit.loss_final < mlp_fit.loss_init / 10
     assert mlp.imbalance < lin.imbalance < radix.imbalance
     assert mlp.fixup_frac < lin.fixup_frac < radix.fixup_frac
+    # the same results, and what the models buy in join time
+    assert res_radix == res_lin == res_mlp
+    assert t_mlp < t_radix / 1.5 and t_lin < t_radix / 1.2, (t_radix, t_lin, t_mlp)
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_sampled_peeling_exact(ctx, torch_mod, sjbmod, alg):
+    # SJB_F_PEEL (no model): heavy keys of the batch's own sample are
+    # aggregated apart; per-window results equal the plain grouped join
+    n, nw = 1 << 16, 6
+    lo = np.arange(nw + 1, dtype=np.uint64) * np.uint64(n)
+    R = np.concatenate([sjbmod.generate_mix(n, w, seed=981 + w) for w in range(nw)])
+    S = np.concatenate([sjbmod.generate_mix(n, w, seed=991 + w) for w in range(nw)])
+    R["key"][::1001] = 0xFFFFFFFF
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    got = {}
+    for f in (F_GROUPED, F_GROUPED | sjbmod.F_PEEL):
+        d_res = torch_mod.zeros(nw * 32, dtype=torch_mod.uint8, device="cuda")
+        plan = ctx.batch_plan(alg, f, dR, lo, dS, lo, d_res)
+        plan.launch()
+        ctx.sync()
+        info = plan.info()
+        assert info.n_heavy == (0 if f == F_GROUPED else info.n_heavy) and (f == F_GROUPED or info.n_heavy > 0)
+        assert info.part_kind == (1 if alg == SMJ else 0)
+        got[f] = d_res.cpu().numpy().view(sjbmod.WRES_DTYPE)[["count", "checksum1"]].copy()
+        plan.destroy()
+    assert np.array_equal(got[F_GROUPED], got[F_GROUPED | sjbmod.F_PEEL])
 
 
 def test_drift_judge_refits_or_falls_back(fitted, torch_mod, sjbmod):
@@ -180,3 +232,25 @@ def test_drift_judge_refits_or_falls_back(fitted, torch_mod, sjbmod):
             plan.destroy()
         assert np.array_equal(got[F_GROUPED | F_LEARNED], got[F_GROUPED])
         assert np.array_equal(got[F_GROUPED | F_LEARNED | F_NO_REFIT], got[F_GROUPED])
+
+
+def test_fit_after_sampled_peeling(sjbmod, torch_mod):
+    # a batch plan with SJB_F_PEEL samples keys before any fit; a later fit
+    # on the same context must still get every fit buffer
+    n, nw = 1 << 16, 2
+    lo = np.arange(nw + 1, dtype=np.uint64) * np.uint64(n)
+    R = np.concatenate([sjbmod.generate_mix(n, w, seed=961 + w) for w in range(nw)])
+    S = np.concatenate([sjbmod.generate_mix(n, w, seed=971 + w) for w in range(nw)])
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    with sjbmod.Context(max_window=n) as c:
+        d_res = torch_mod.zeros(nw * 32, dtype=torch_mod.uint8, device="cuda")
+        plan = c.batch_plan(HJ, F_GROUPED | sjbmod.F_PEEL, dR, lo, dS, lo, d_res)
+        plan.launch()
+        c.sync()
+        assert plan.info().n_heavy > 0
+        plan.destroy()
+        rep = c.rmi_fit(dR, len(R), root=sjbmod.RMI_MLP, steps=50, lr=0.01)
+        assert rep.fallback == 0 and rep.loss_final < rep.loss_init
+        st = c.lazy_join(SMJ, 0, F_GROUPED | F_LEARNED, dR, n, dS, n)
+        st2 = c.lazy_join(SMJ, 0, F_GROUPED, dR, n, dS, n)
+        assert (st.result_count, st.checksum1) == (st2.result_count, st2.checksum1)
