@@ -121,7 +121,8 @@ static bool valid_method(int m) { return m >= SJB_OUT_NOOUTPUT && m <= SJB_OUT_A
 // partition pass (plus the range parameters or the heavy-key reset it
 // needs).  Both sides non-empty.
 int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
-                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out, int buf) {
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out, int buf,
+                     sjb_window_result_t *direct) {
     WindowPlan &p = *out;
     p.ps = buf < 0 ? c->stream : c->ov_stream;
     p.part = buf < 0 ? c->d_part : c->ov_part[buf];
@@ -139,6 +140,8 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.ncL = int((nL + kChunk - 1) / kChunk);
     p.ncR = int((nR + kChunk - 1) / kChunk);
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
+    // the count paths that k_partition_pipe partitions sum in the join
+    p.direct = SJB_PART_PIPE && direct && !write && !p.peel && p.kind != PART_LEARNED ? direct : nullptr;
     int rc;
     if (p.kind == PART_RANGE) {
         rc = launch_minmax(c, p.ps, L, nL, R, nR, p.logF);
@@ -203,7 +206,7 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     if (buf >= 0) SJB_HIP_TRY(hipStreamWaitEvent(ps, c->ov_join_done[buf], 0));
     if (ev) SJB_HIP_TRY(hipEventRecord(ev[0], ps));
     WindowPlan p;
-    rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p, buf);
+    rc = window_partition(c, alg, flags, L, nL, R, nR, write, &p, buf, d_res);
     if (rc) return rc;
     *logF_used = p.logF;
     if (ev) SJB_HIP_TRY(hipEventRecord(ev[1], ps));
@@ -217,9 +220,10 @@ int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, ui
     rc = alg == SJB_ALG_HJ ? launch_hj_join(c, p, L, R, out, out_cap, wrap)
                            : launch_smj_join(c, p, L, R, out, out_cap, wrap);
     if (rc) return rc;
-    rc = launch_reduce(c, 1 << p.logF, d_res, p.peel ? c->rmi_nheavy : 0,
-                       p.heavy);
-    if (rc) return rc;
+    if (!p.direct) {
+        rc = launch_reduce(c, 1 << p.logF, d_res, p.peel ? c->rmi_nheavy : 0, p.heavy);
+        if (rc) return rc;
+    }
     if (ev) SJB_HIP_TRY(hipEventRecord(ev[3], c->stream));
     // buffer set 0 is also the ctx's own: its next overlapped partition pass
     // waits for this join

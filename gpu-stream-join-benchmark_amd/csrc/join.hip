@@ -1295,7 +1295,8 @@ __host__ __device__ constexpr int idcap_of() {
 // one window: one workgroup per partition
 // ---------------------------------------------------------------------------
 // The workgroup's (count, C1, C2) goes to partials[3p ...] and is summed
-// by k_reduce.
+// by k_reduce, or, when `direct` is set (the partition pass zeroed it), is
+// added to the window result with three 64-bit atomics.
 template <int ALG, bool C2, bool WRITE, bool GROUPED>
 // Count bodies whose LDS allows three workgroups per CU are held to 80
 // VGPRs (six waves per SIMD) to match.
@@ -1303,7 +1304,8 @@ __global__ __launch_bounds__(kJoinThreads)
 __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 : 6))) void k_join(
     JoinIO io, const uint32_t *__restrict__ offs,
                                                        int logF, int ncL, int ncR, uint32_t nL,
-                                                       int bshift, uint64_t *__restrict__ partials) {
+                                                       int bshift, uint64_t *__restrict__ partials,
+                                                       sjb_window_result_t *__restrict__ direct) {
     __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, WRITE>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
@@ -1334,7 +1336,11 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
         uint64_t t = 0;
 #pragma unroll
         for (int w = 0; w < kW; w++) t += s_red[threadIdx.x * kW + w];
-        partials[3 * p + threadIdx.x] = t;
+        // sums mod 2^64: the atomics give the same bits in any order
+        if (!direct)
+            partials[3 * p + threadIdx.x] = t;
+        else if (t)
+            atomicAdd(reinterpret_cast<unsigned long long *>(direct) + threadIdx.x, (unsigned long long)t);
     }
 }
 
@@ -1490,7 +1496,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
 #define SJB_J(C2_, W_, G_)                                                                          \
     k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL,      \
                                                                  p.ncR, uint32_t(p.nL), bshift,    \
-                                                                 c->d_partials)
+                                                                 c->d_partials, p.direct)
     if (p.payload == PAY_IDX) {
         if (c2)
             SJB_J(true, true, false);

@@ -570,9 +570,12 @@ template <int KIND, int PAY>
 __global__ __launch_bounds__(kPartThreads) void k_partition_pipe(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
     uint64_t nR, int ncR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out) {
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, sjb_window_result_t *__restrict__ direct) {
     extern __shared__ uint32_t lds[];
     const int F = 1 << logF;
+    // the join kernel accumulates into `direct`; it starts from zero here
+    if (direct && blockIdx.x == 0 && threadIdx.x < 4)
+        reinterpret_cast<uint64_t *>(direct)[threadIdx.x] = 0;
     uint32_t *hist = lds;       // F + 1 counters
     uint32_t *ws = lds + F + 2; // wave sums
     Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 2 + kPartThreads / kWave + 1) & ~1));
@@ -745,7 +748,7 @@ static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
         if (p.payload == PAY_VAL) {
             const int grid = std::min(nblk, std::max(c->n_cu, 1));
             k_partition_pipe<KIND, PAY_VAL><<<grid, kPartThreads, lds, p.ps>>>(
-                L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part);
+                L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.direct);
             SJB_HIP_TRY(hipGetLastError());
             return SJB_OK;
         }

@@ -163,6 +163,10 @@ struct WindowPlan {
     Rec *part;
     uint32_t *offs;
     uint64_t *heavy;
+    // non-null: the partition pass zeroes this result and every join
+    // workgroup adds its (count, C1, C2) to it with 64-bit atomics, so no
+    // reduction kernel runs (count paths partitioned by k_partition_pipe)
+    sjb_window_result_t *direct;
 };
 
 }  // namespace sjb
@@ -298,7 +302,8 @@ int launch_part_sizes(hipStream_t s, const uint32_t *offs, int logF, int ncL, in
 
 // ---- api.hip ----
 int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
-                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *p, int buf = -1);
+                     const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *p, int buf = -1,
+                     sjb_window_result_t *direct = nullptr);  // direct: see WindowPlan::direct
 int window_enqueue(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                    const sjb_tuple_t *R, uint64_t nR, int write, sjb_result_tuple_t *out,
                    uint64_t out_cap, int wrap, sjb_window_result_t *d_res, int *logF_used,
