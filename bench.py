@@ -755,7 +755,7 @@ def run_slide(a, np, torch, sjb):
 # ---------------------------------------------------------------------------
 # eager per-batch joins (only with --workload eager)
 # ---------------------------------------------------------------------------
-def bench_eager(a):
+def run_eager(a):
     """Eager (per-batch) joins on host batches, as the eager executors
     receive them: each batch is copied to the device, probes the other
     side's resident window state and is appended to its own side; the
@@ -817,6 +817,16 @@ def bench_eager(a):
                       "p50_batch_ms": float(np.percentile(lat_ms, 50)),
                       "p99_batch_ms": float(np.percentile(lat_ms, 99)), "matches": int(matches)}
         log(f"eager {name}: {rows[name]}")
+    return rows
+
+
+def bench_eager(a):
+    """--workload eager: the eager cases of run_eager as one JSON line (value:
+    HJ with 2^14-tuple batches), with the oracle's eager hash join as the
+    CPU baseline."""
+    import sjb
+
+    rows = run_eager(a)
     hj = rows["HJ"]
     out = {
         "metric": "eager per-batch join: input tuples/s and p50/p99 per-batch latency (host batches)",
@@ -924,7 +934,8 @@ def main():
             ("nlj", lambda: run_nlj(a, np, torch, sjb, a.extra_steps, 1)),
             ("batch", lambda: run_batch(a, np, torch, sjb, thetas, ["HJ", "SMJ"], a.extra_steps, 1)),
             ("learned", lambda: run_learned(a, np, torch, sjb, a.extra_steps, 1)),
-            ("slide", lambda: run_slide(a, np, torch, sjb))]
+            ("slide", lambda: run_slide(a, np, torch, sjb)),
+            ("eager", lambda: run_eager(a))]
     for name, fn in legs:
         t0 = time.time()
         try:
