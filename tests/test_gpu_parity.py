@@ -341,3 +341,26 @@ def test_pinned_ring_slots_and_output(ctx, sjbmod, orc):
 def to_dev_np(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to("cuda")
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_async_result_slots_start_dirty(ctx, torch_mod, orc, sjbmod, alg):
+    """The count paths sum into the caller's result slot (the partition
+    pass zeroes it, each join workgroup adds its share): slots full of stale
+    bytes, with several windows of different sizes in flight on the stream,
+    still get exact count / C1 / C2."""
+    sizes = [(1 << 16, 1 << 16), (70000, 65536), (5000, 4096), (1 << 20, 1 << 20)]
+    wins = [(make(nl, 1 << 16, 300 + i), make(nr, 1 << 16, 400 + i)) for i, (nl, nr) in enumerate(sizes)]
+    refs = [orc.lazy_hj(R, S, 1 << 16)[0] for R, S in wins]
+    dev = [(to_dev(torch_mod, R), to_dev(torch_mod, S)) for R, S in wins]
+    for flags in (F_C2, F_GROUPED, 0):
+        print(f"case dirty slots alg={alg} flags={flags}", flush=True)
+        d_res = torch_mod.full((32 * len(wins),), 0xAB, dtype=torch_mod.uint8, device="cuda")
+        for i, ((dR, dS), (R, S)) in enumerate(zip(dev, wins)):
+            ctx.lazy_join_async(alg, flags, dR, len(R), dS, len(S), d_res.data_ptr() + 32 * i)
+        ctx.sync()
+        got = d_res.cpu().numpy().view(sjbmod.WRES_DTYPE)
+        for i, o in enumerate(refs):
+            assert (int(got[i]["count"]), int(got[i]["checksum1"])) == (o.count, o.c1), (flags, i)
+            if flags & F_C2:
+                assert int(got[i]["checksum2"]) == o.c2, (flags, i)
