@@ -258,6 +258,24 @@ def test_full_size_window(ctx, torch_mod, orc, sjbmod):
     assert 0.9 * 2**20 < o.count < 1.1 * 2**20
 
 
+def test_window_above_2_22(torch_mod, orc, sjbmod):
+    # 2^23 + 12345 / 2^23 - 777 tuples per side (keys uniform in [0, 2^25)):
+    # HJ at fanout 2^12 with half-full tiles and SMJ at 2^12, 513 / 512
+    # chunks per side, so the join builds its run table in several passes
+    # (the general view path) at full scale.
+    nL, nR = (1 << 23) + 12345, (1 << 23) - 777
+    print(f"case window above 2^22 nL={nL} nR={nR}", flush=True)
+    R = sjbmod.generate(nL, sjbmod.DIST_UNIFORM, 1 << 25, 0.0, seed=31)
+    S = sjbmod.generate(nR, sjbmod.DIST_UNIFORM, 1 << 25, 0.0, seed=32)
+    o, _ = orc.lazy_hj(R, S, 1 << 20)
+    with sjbmod.Context(device=0, max_window=1 << 24, distinct_keys=1 << 25) as c:
+        for alg in (HJ, SMJ):
+            st = run_count(c, torch_mod, alg, R, S)
+            assert (st.result_count, st.checksum1, st.checksum2) == o.triple(), alg
+            assert st.fanout_log2 == 12, alg
+    assert 0.9 * 2**21 < o.count < 1.1 * 2**21
+
+
 def test_ragged_multichunk_windows(ctx, torch_mod, orc, sjbmod):
     # more 16384-tuple chunks than partition workgroups (one per CU), both
     # sides ending in a partial chunk and of different lengths: the
