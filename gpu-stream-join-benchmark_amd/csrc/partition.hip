@@ -727,9 +727,16 @@ int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias) {
     return std::min(std::max(logF, 0), kMaxLogF);
 }
 
+// Fanout of a lone window: the tile rule above, raised to at least 2^9
+// partitions (two join workgroups per CU) while partitions keep >= 64 build
+// records — a small window, or one with few distinct keys and many matches,
+// would otherwise run its join on a few dozen workgroups.
 int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested) {
     if (requested > 0) return std::min(std::max(requested, 1), kMaxLogF);
-    return std::max(batch_log_fanout(alg, flags, n_build, 0), 1);
+    const int logF = std::max(batch_log_fanout(alg, flags, n_build, 0), 1);
+    int floor_logF = 9;
+    while (floor_logF > logF && (n_build >> floor_logF) < 64) floor_logF--;
+    return std::max(logF, floor_logF);
 }
 
 size_t partition_lds_bytes_chunk(int logF, int chunk) {

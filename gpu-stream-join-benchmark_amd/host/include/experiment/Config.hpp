@@ -50,4 +50,10 @@ struct ExperimentConfig {
     bool measurePower = false;
     bool extendedCsv = false;      // append count/checksum/throughput columns
     bool searchRate = true;        // false: run dataSize once (no rate search)
+    // bound on the tuples per source stream (both streams are held in host
+    // memory for the run): the rate search stops there and reports a capped
+    // result; a run above it is refused.  2^30 tuples (2 x 16 GiB) is the
+    // largest size this driver has run on the MI355X host; a search step at
+    // 2^31 ended the process there, for a reason not found.
+    uint64_t maxDataSize = uint64_t(1) << 30;
 };

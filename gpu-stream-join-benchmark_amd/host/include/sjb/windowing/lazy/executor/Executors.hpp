@@ -38,11 +38,16 @@ protected:
     std::vector<ResultTuple> scratch;  // exact-size buffer when the sink ring is too small
 };
 
+// NoOutput executors report the match count (and C1): grouped mode gives
+// both in O(n) whatever the keys' multiplicity (the reference's SMJ count is
+// the same closed form, sum over keys of L_k * R_k, GPU_SMJ.cu:34-44); the
+// other methods enumerate every pair and add C2.
 #define SJB_GPU_LAZY_3(NAME, ALG, METHOD)                                                   \
     class NAME : public GPU_Lazy {                                                          \
     public:                                                                                 \
         NAME(uint64_t numDistinctKeys, uint64_t batchSize, uint64_t maxTupleInWindow)       \
-            : GPU_Lazy(ALG, METHOD, numDistinctKeys, batchSize, maxTupleInWindow) {}        \
+            : GPU_Lazy(ALG, METHOD, numDistinctKeys, batchSize, maxTupleInWindow, 0.0,      \
+                       (METHOD) == SJB_OUT_NOOUTPUT ? SJB_F_GROUPED : SJB_F_C2) {}          \
     };
 
 // HJ (src/sjb/windowing/lazy/executor/GPU/GPU_HJ_*.cu)

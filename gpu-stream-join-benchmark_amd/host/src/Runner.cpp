@@ -7,6 +7,8 @@
 #include <fstream>
 #include <iostream>
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -27,6 +29,9 @@ std::string Runner::csvHeader(bool extended) {
 // Runner.cpp:14-104
 Runner::RunResult Runner::runExperimentWithConfig(ExperimentConfig config, bool trackProgress) {
     const DataConfig &d = config.dataConfig;
+    if (config.maxDataSize && d.dataSize > config.maxDataSize)
+        throw std::invalid_argument("data size " + std::to_string(d.dataSize) + " tuples per stream exceeds maxDataSize " +
+                                    std::to_string(config.maxDataSize));
     // page-locked source streams (Runner.cpp:18-33), so GPU executors copy
     // batches and windows from them asynchronously
     sjb_host::PinnedArray<Tuple> left(d.dataSize), right(d.dataSize);
@@ -118,6 +123,14 @@ void Runner::runExperimentsAndWriteResultToFile(std::vector<ExperimentConfig> co
             if (median < 1000) {
                 latestSustained = currentRate;
                 nextRate = latestUnsustained == 0 ? nextRate * 2 : (latestUnsustained + currentRate) / 2;
+                if (cur.maxDataSize && nextRate > cur.maxDataSize) {
+                    if (currentRate >= cur.maxDataSize) {
+                        fprintf(stderr, "[runner] sustained at maxDataSize (%llu tuples per stream): search capped\n",
+                                (unsigned long long)cur.maxDataSize);
+                        break;
+                    }
+                    nextRate = cur.maxDataSize;
+                }
             } else {
                 latestUnsustained = currentRate;
                 nextRate = (latestSustained + currentRate) / 2;

@@ -6,7 +6,8 @@
 // Optional keys beyond the reference's: dataSizeTuples (initial data size
 // per stream), precisionTuples, sourceThreads, extendedCsv ("Yes" appends
 // match_count, checksum1, input_tuples_per_s), rateSearch ("No" runs the
-// initial data size once), usePK ("Yes": persistent-kernel eager executor).
+// initial data size once), usePK ("Yes": persistent-kernel eager executor),
+// maxDataSizeTuples (bound on the tuples per stream, see ExperimentConfig).
 #include <algorithm>
 #include <cstdio>
 #include <exception>
@@ -104,6 +105,7 @@ int main(int argc, char **argv) {
         base.algorithmConfig.eagerJoinConfig.usePK = yesno(opt(y, "usePK", "No"), "usePK");
         if (y.count("dataSizeTuples")) base.dataConfig.dataSize = std::stoull(opt(y, "dataSizeTuples", "0"));
         if (y.count("precisionTuples")) base.precision = std::stoull(opt(y, "precisionTuples", "0"));
+        if (y.count("maxDataSizeTuples")) base.maxDataSize = std::stoull(opt(y, "maxDataSizeTuples", "0"));
         if (y.count("sourceThreads")) base.engineConfig.nSourceThreads = std::stoull(opt(y, "sourceThreads", "1"));
         const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
 

@@ -234,3 +234,23 @@ def test_driver_gpu_power(tmp_path, sjbmod, orc):
     ts, w = zip(*[(int(a), float(b)) for a, b in (ln.split(",") for ln in lines)])
     assert all(10.0 <= x <= 3000.0 for x in w), w[:10]
     assert list(ts) == sorted(ts)
+
+
+def test_driver_rate_search_capped(tmp_path):
+    # the rate search doubles the stream size while runs stay under 1 s; at
+    # maxDataSizeTuples it stops and reports that size (the CPU hash join
+    # runs 2^16-tuple streams far below 1 s)
+    cap = 1 << 16
+    y = yaml_cfg("HJ", "CPU", "LAZY", "NoOutput", data=1 << 14).replace('rateSearch: "No"', 'rateSearch: "Yes"')
+    y += f"precisionTuples: 1024\nmaxDataSizeTuples: {cap}\n"
+    rows = run_driver(tmp_path, y)
+    assert len(rows) == 1 and int(rows[0]["ingestion_rate"]) == cap
+
+
+def test_driver_refuses_stream_above_cap(tmp_path):
+    y = yaml_cfg("HJ", "CPU", "LAZY", "NoOutput", data=1 << 17) + "maxDataSizeTuples: 65536\n"
+    cfg = tmp_path / "cfg.yaml"
+    cfg.write_text(y)
+    p = subprocess.run([os.path.join(BIN, "runExperiment"), str(cfg), str(tmp_path / "o.csv")],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert p.returncode != 0 and "exceeds maxDataSize" in p.stderr
