@@ -431,10 +431,10 @@ def run_nlj(a, np, torch, sjb, steps, warmup):
         "input_tuples_per_s": 2 * N * steps / t_wall,
         "window_ms": {"p50": float(np.percentile(win_ms, 50)), "p99": float(np.percentile(win_ms, 99)),
                       "mean": float(win_ms.mean())},
-        "roofline": {"bound": "valu", "kernel": "NLJ window (key compaction + compare + reduction)",
+        "roofline": {"bound": "valu", "kernel": "NLJ window (key compaction + compare; workgroups sum into the result)",
                      "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12, "unit": "T compares/s",
                      "frac": achieved / VALU_PEAK_OPS, "traffic": None,
-                     "note": "one int32 compare per pair against one VALU lane-op per clock",
+                     "note": "one int32 compare (one v_cmpx_ne_u32 lane) per pair against one VALU lane-op per clock",
                      "ops_per_launch": compares * NLJ_OPS_PER_COMPARE, "avg_launch_ms": 1e3 * avg},
         "oracle_check": {"windows_checked": [0, nw - 1], "equal": all(check)},
         "total_matches": int(last["count"].sum()),

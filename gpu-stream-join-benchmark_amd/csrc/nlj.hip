@@ -9,12 +9,14 @@
 // Count kernel: each thread keeps 4 right (probe) keys in registers; the
 // left keys of the workgroup's range are compacted once into a 4-B key array
 // and come in through scalar loads (16 keys per instruction), shared by the
-// whole wave, straight into the compare's scalar operand.  The compare is
-// VALU work: x = lk ^ sk and m = min3(m, x0, x1) — 1.5 ops per compare — so
-// a tile with no match costs nothing else.  Only waves with a lane whose m
-// reached 0 rescan the tile (again 32 keys per load batch, a cheap min-xor
-// test per key) and add count, C1 and C2 per matching pair.  The NLJ is
-// VALU-bound (n_L*n_R compares), not HBM-bound.
+// whole wave, straight into the compare's scalar operand.  Each compare is
+// one VALU instruction, v_cmpx_ne_u32, which takes the lanes holding an
+// equal key out of EXEC; the lanes gone after a block of 64 compares are the
+// lanes with a match, so a tile with no match costs nothing else.  Only
+// waves with such a lane rescan the tile (32 keys per load batch, a cheap
+// min-xor test per key) and add count, C1 and C2 per matching pair.  The NLJ
+// is VALU-bound (n_L*n_R compares), not HBM-bound.  Workgroups add their
+// sums to the window result with 64-bit atomics (k_nlj_keys zeroes it).
 //
 // Bitmap method: one wave holds 64 consecutive right keys; for each left
 // key, __ballot(lk == rk) is exactly the 64-bit match word of that left
@@ -32,11 +34,14 @@ constexpr int kNljThreads = 256;
 #ifndef SJB_NLJ_PER
 #define SJB_NLJ_PER 4
 #endif
+// 128-key tiles and 16 workgroups per CU (two rounds of the 8 that fit, so
+// early finishers take more work) measured best at 2^16 x 2^16 per window:
+// 32.1 T compares/s against 29.2 T with 256-key tiles and 8 per CU.
 #ifndef SJB_NLJ_LTILE
-#define SJB_NLJ_LTILE 256
+#define SJB_NLJ_LTILE 128
 #endif
 #ifndef SJB_NLJ_WG_PER_CU
-#define SJB_NLJ_WG_PER_CU 8
+#define SJB_NLJ_WG_PER_CU 16
 #endif
 constexpr int kNljPer = SJB_NLJ_PER;  // right keys per thread
 constexpr int kNljRTile = kNljThreads * kNljPer;  // right tuples per workgroup
@@ -76,10 +81,56 @@ __device__ __forceinline__ void nlj_tile_hits(const uint32_t *__restrict__ tk, i
     }
 }
 
+// Lanes of the wave holding a right key equal to one of 16 left keys (kc,
+// scalar registers).  Each compare is one v_cmpx_ne_u32, which drops from
+// EXEC every lane whose key equals the left key, so 16 x kNljPer compares
+// cost as many VALU instructions and nothing else: the lanes gone from EXEC
+// at the end are the lanes with a match.  EXEC is saved and restored inside
+// the one asm statement, so no compiler-generated instruction runs under the
+// reduced mask.  v_cmpx also writes VCC (clobbered).  s_nop 4 gives the EXEC
+// write of the last VALU compare five wait states before the scalar read of
+// EXEC.
+#define SJB_CMPX4(k, a, b, c, d)                        \
+    "v_cmpx_ne_u32_e32 vcc, %[k" #k "], %[r" #a "]\n\t" \
+    "v_cmpx_ne_u32_e32 vcc, %[k" #k "], %[r" #b "]\n\t" \
+    "v_cmpx_ne_u32_e32 vcc, %[k" #k "], %[r" #c "]\n\t" \
+    "v_cmpx_ne_u32_e32 vcc, %[k" #k "], %[r" #d "]\n\t"
+#if SJB_NLJ_PER == 4
+#define SJB_CMPX(k) SJB_CMPX4(k, 0, 1, 2, 3)
+#define SJB_CMPX_RIGHT [r0] "v"(sk[0]), [r1] "v"(sk[1]), [r2] "v"(sk[2]), [r3] "v"(sk[3])
+#elif SJB_NLJ_PER == 8
+#define SJB_CMPX(k) SJB_CMPX4(k, 0, 1, 2, 3) SJB_CMPX4(k, 4, 5, 6, 7)
+#define SJB_CMPX_RIGHT                                                                            \
+    [r0] "v"(sk[0]), [r1] "v"(sk[1]), [r2] "v"(sk[2]), [r3] "v"(sk[3]), [r4] "v"(sk[4]), \
+        [r5] "v"(sk[5]), [r6] "v"(sk[6]), [r7] "v"(sk[7])
+#else
+#error "SJB_NLJ_PER must be 4 or 8"
+#endif
+__device__ __forceinline__ uint64_t nlj_match16(const uint32_t (&kc)[16], const uint32_t (&sk)[kNljPer]) {
+    uint64_t saved, gone;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        SJB_CMPX(0) SJB_CMPX(1) SJB_CMPX(2) SJB_CMPX(3) SJB_CMPX(4) SJB_CMPX(5) SJB_CMPX(6) SJB_CMPX(7)
+        SJB_CMPX(8) SJB_CMPX(9) SJB_CMPX(10) SJB_CMPX(11) SJB_CMPX(12) SJB_CMPX(13) SJB_CMPX(14) SJB_CMPX(15)
+        "s_nop 4\n\t"
+        "s_andn2_b64 %[gone], %[sv], exec\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [sv] "=&s"(saved), [gone] "=&s"(gone)
+        : [k0] "s"(kc[0]), [k1] "s"(kc[1]), [k2] "s"(kc[2]), [k3] "s"(kc[3]), [k4] "s"(kc[4]),
+          [k5] "s"(kc[5]), [k6] "s"(kc[6]), [k7] "s"(kc[7]), [k8] "s"(kc[8]), [k9] "s"(kc[9]),
+          [k10] "s"(kc[10]), [k11] "s"(kc[11]), [k12] "s"(kc[12]), [k13] "s"(kc[13]), [k14] "s"(kc[14]),
+          [k15] "s"(kc[15]), SJB_CMPX_RIGHT
+        : "vcc", "scc");
+    return gone;
+}
+#undef SJB_CMPX
+#undef SJB_CMPX4
+#undef SJB_CMPX_RIGHT
+
 template <bool C2, bool WRITE>
 __global__ __launch_bounds__(kNljThreads) void k_nlj(
     const uint32_t *__restrict__ lkeys, const sjb_tuple_t *__restrict__ L, uint64_t nL,
-    const sjb_tuple_t *__restrict__ R, uint64_t nR, uint64_t l_split, uint64_t *__restrict__ partials,
+    const sjb_tuple_t *__restrict__ R, uint64_t nR, uint64_t l_split, sjb_window_result_t *__restrict__ res,
     sjb_result_tuple_t *__restrict__ out, uint64_t out_cap, int wrap, unsigned long long *__restrict__ cursor) {
     __shared__ uint64_t s_red[3 * kNljThreads / kWave];
     __shared__ uint32_t s_ws[kNljThreads / kWave];
@@ -105,42 +156,35 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
         const int tn = int(min<uint64_t>(kNljLTile, lend - t0));
         // Left keys are the same for every lane: they come in through scalar
         // loads (16 per instruction) straight into the compare's scalar
-        // operand.  Two running minima (independent chains); min(min(m, x0),
-        // x1) folds into one v_min3 per two compares.
+        // operand (nlj_match16).
         const uint32_t *__restrict__ tk = lkeys + t0;
-        uint32_t m = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu;
         const int t16 = tn & ~31;
         // software-pipelined key loads: the next 16 keys are requested
         // before the current 16 are compared (the reads past t16 stay in
         // the key array's slack and are not compared)
         uint32_t kc[16], kn[16];
+        uint64_t hit = 0;  // lanes with a match in this tile (wave-uniform)
 #pragma unroll
         for (int c = 0; c < 16; c++) kc[c] = tk[c];
-        for (int e = 0; e < t16; e += 16) {
+        for (int e = 0; e < t16; e += 32) {  // t16 is a multiple of 32
             __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): kc has landed
 #pragma unroll
             for (int c = 0; c < 16; c++) kn[c] = tk[e + 16 + c];
+            hit |= nlj_match16(kc, sk);
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // kn has landed
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                const uint32_t lk = kc[c];
-#pragma unroll
-                for (int i = 0; i < kNljPer; i += 4) {
-                    m = min(min(m, lk ^ sk[i]), lk ^ sk[i + 1]);
-                    m2 = min(min(m2, lk ^ sk[i + 2]), lk ^ sk[i + 3]);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 16; c++) kc[c] = kn[c];
+            for (int c = 0; c < 16; c++) kc[c] = tk[e + 32 + c];
+            hit |= nlj_match16(kn, sk);
         }
+        uint32_t m = 0xFFFFFFFFu;
         for (int e = t16; e < tn; e++) {
             const uint32_t lk = tk[e];
 #pragma unroll
             for (int i = 0; i < kNljPer; i++) m = min(m, lk ^ sk[i]);
         }
-        m = min(m, m2);
-        // Exact pass only for lanes that saw a zero xor.
+        // Exact pass only for lanes with an equal key.
         uint32_t mine = 0;
-        if (m == 0) {
+        if (((hit >> __lane_id()) & 1) || m == 0) {
             nlj_tile_hits(tk, tn, sk, valid, [&](int e, int i) {
                 mine++;
                 if (!WRITE) {
@@ -183,16 +227,21 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
         }
     }
     const Acc r = block_sum(acc, s_red);
+    // the window result was zeroed by k_nlj_keys; only non-zero sums are added
     if (threadIdx.x == 0) {
-        const uint64_t b = uint64_t(blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[3 * b] = r.count;
-        partials[3 * b + 1] = r.c1;
-        partials[3 * b + 2] = r.c2;
+        auto *d = reinterpret_cast<unsigned long long *>(res);
+        if (r.count) atomicAdd(d, (unsigned long long)r.count);
+        if (r.c1) atomicAdd(d + 1, (unsigned long long)r.c1);
+        if (r.c2) atomicAdd(d + 2, (unsigned long long)r.c2);
     }
 }
 
 // Keys of n tuples, compacted (4 B each) for the scalar loads of k_nlj.
-__global__ void k_nlj_keys(const sjb_tuple_t *__restrict__ L, uint64_t n, uint32_t *__restrict__ keys) {
+// It also zeroes the window result that k_nlj's workgroups add into.
+__global__ void k_nlj_keys(const sjb_tuple_t *__restrict__ L, uint64_t n, uint32_t *__restrict__ keys,
+                           sjb_window_result_t *__restrict__ res) {
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(sjb_window_result_t) / 8)
+        reinterpret_cast<uint64_t *>(res)[threadIdx.x] = 0;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         keys[i] = L[i].key;
 }
@@ -201,8 +250,8 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
                uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
                sjb_window_result_t *d_out) {
     const uint64_t gx = (nR + kNljRTile - 1) / kNljRTile;
-    // Split the left side so the grid fills the chip (8 workgroups per CU)
-    // without exceeding the partials buffer.
+    // Split the left side so the grid fills the chip (SJB_NLJ_WG_PER_CU
+    // workgroups per CU).
     uint64_t want = uint64_t(c->n_cu) * SJB_NLJ_WG_PER_CU;
     uint64_t gy = (want + gx - 1) / gx;
     uint64_t max_gy = (nL + kNljLTile - 1) / kNljLTile;
@@ -226,25 +275,25 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
         if (rc) return rc;
     }
     const uint32_t *lkeys = c->d_nlj_keys;
-    k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys);
+    k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys, d_out);
     SJB_HIP_TRY(hipGetLastError());
     if (write) {
         if (c2)
-            k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, d_out,
                                                                    out, out_cap, wrap, cur);
         else
-            k_nlj<false, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<false, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, d_out,
                                                                     out, out_cap, wrap, cur);
     } else {
         if (c2)
-            k_nlj<true, false><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, c->d_partials,
+            k_nlj<true, false><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, d_out,
                                                                     out, out_cap, wrap, cur);
         else
             k_nlj<false, false><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split,
-                                                                     c->d_partials, out, out_cap, wrap, cur);
+                                                                     d_out, out, out_cap, wrap, cur);
     }
     SJB_HIP_TRY(hipGetLastError());
-    return launch_reduce(c, int(gx * gy), d_out, 0);
+    return SJB_OK;
 }
 
 // ---------------------------------------------------------------------------
