@@ -18,11 +18,13 @@
 // is VALU-bound (n_L*n_R compares), not HBM-bound.  Workgroups add their
 // sums to the window result with 64-bit atomics (k_nlj_keys zeroes it).
 //
-// Bitmap method: one wave holds 64 consecutive right keys; for each left
-// key, __ballot(lk == rk) is exactly the 64-bit match word of that left
-// tuple (the reference builds 32-bit words with one atomicAdd each and
-// decodes them on the host).  Non-zero words are appended as (l, w, mask)
-// triples and decoded on the device into ResultTuple rows.
+// Bitmap method: each row of a wave holds 64 consecutive right keys; for a
+// left key, __ballot(lk == rk) over a row is exactly the 64-bit match word
+// of that left tuple (the reference builds 32-bit words with one atomicAdd
+// each and decodes them on the host).  The count kernel's v_cmpx prefilter
+// finds the tiles with a match; only those are rescanned with ballots, and
+// non-zero words are appended as (l, w, mask) triples and decoded on the
+// device into ResultTuple rows.
 #include <algorithm>
 
 #include "sjb_device.hpp"
@@ -236,19 +238,22 @@ __global__ __launch_bounds__(kNljThreads) void k_nlj(
     }
 }
 
-// Keys of n tuples, compacted (4 B each) for the scalar loads of k_nlj.
-// It also zeroes the window result that k_nlj's workgroups add into.
+// Keys of n tuples, compacted (4 B each) for the scalar loads of k_nlj and
+// k_nlj_bitmap.  It also zeroes the window result that k_nlj's workgroups
+// add into (res non-null).
 __global__ void k_nlj_keys(const sjb_tuple_t *__restrict__ L, uint64_t n, uint32_t *__restrict__ keys,
                            sjb_window_result_t *__restrict__ res) {
-    if (blockIdx.x == 0 && threadIdx.x < sizeof(sjb_window_result_t) / 8)
+    if (res && blockIdx.x == 0 && threadIdx.x < sizeof(sjb_window_result_t) / 8)
         reinterpret_cast<uint64_t *>(res)[threadIdx.x] = 0;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         keys[i] = L[i].key;
 }
 
-int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
-               uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
-               sjb_window_result_t *d_out) {
+// Grid of k_nlj / k_nlj_bitmap (x: right tiles of kNljRTile tuples, y: left
+// ranges of l_split keys, a multiple of kNljLTile) and the compacted left
+// keys of the window (k_nlj_keys, which zeroes *res when res is non-null).
+static int nlj_prepare(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, uint64_t nR, sjb_window_result_t *res,
+                       dim3 *grid, uint64_t *l_split_out) {
     const uint64_t gx = (nR + kNljRTile - 1) / kNljRTile;
     // Split the left side so the grid fills the chip (SJB_NLJ_WG_PER_CU
     // workgroups per CU).
@@ -262,21 +267,33 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
     uint64_t l_split = (nL + gy - 1) / gy;
     l_split = (l_split + kNljLTile - 1) / kNljLTile * kNljLTile;
     gy = (nL + l_split - 1) / l_split;
-    const dim3 grid((unsigned)gx, (unsigned)gy);
-    const bool c2 = (flags & SJB_F_C2) != 0;
-    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+    *grid = dim3((unsigned)gx, (unsigned)gy);
+    *l_split_out = l_split;
     if (c->nlj_keys_cap < nL) {  // grow-only key buffer, owned by the ctx
-        const uint64_t want = std::max<uint64_t>(nL, c->cap);
+        const uint64_t want_keys = std::max<uint64_t>(nL, c->cap);
         void *p = c->d_nlj_keys;
         int rc = dev_regrow(c, &p, c->d_nlj_keys ? (c->nlj_keys_cap + kNljKeySlack) * sizeof(uint32_t) : 0,
-                            (want + kNljKeySlack) * sizeof(uint32_t));
+                            (want_keys + kNljKeySlack) * sizeof(uint32_t));
         c->d_nlj_keys = static_cast<uint32_t *>(p);
-        c->nlj_keys_cap = c->d_nlj_keys ? want : 0;
+        c->nlj_keys_cap = c->d_nlj_keys ? want_keys : 0;
         if (rc) return rc;
     }
-    const uint32_t *lkeys = c->d_nlj_keys;
-    k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys, d_out);
+    k_nlj_keys<<<unsigned(std::min<uint64_t>((nL + 255) / 256, 4096)), 256, 0, c->stream>>>(L, nL, c->d_nlj_keys,
+                                                                                          res);
     SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
+int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R, uint64_t nR,
+               uint32_t flags, int write, sjb_result_tuple_t *out, uint64_t out_cap, int wrap,
+               sjb_window_result_t *d_out) {
+    dim3 grid;
+    uint64_t l_split;
+    int rc = nlj_prepare(c, L, nL, nR, d_out, &grid, &l_split);
+    if (rc) return rc;
+    const bool c2 = (flags & SJB_F_C2) != 0;
+    auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
+    const uint32_t *lkeys = c->d_nlj_keys;
     if (write) {
         if (c2)
             k_nlj<true, true><<<grid, kNljThreads, 0, c->stream>>>(lkeys, L, nL, R, nR, l_split, d_out,
@@ -299,36 +316,71 @@ int launch_nlj(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t 
 // ---------------------------------------------------------------------------
 // Bitmap method
 // ---------------------------------------------------------------------------
-constexpr int kBmThreads = 256;  // 4 waves = 4 right words of 64 keys
-constexpr int kBmLTile = 2048;
-
-__global__ __launch_bounds__(kBmThreads) void k_nlj_bitmap(
-    const sjb_tuple_t *__restrict__ L, uint64_t nL, const sjb_tuple_t *__restrict__ R, uint64_t nR,
+// Masks of the Bitmap method, on the count kernel's layout: each lane holds
+// kNljPer right keys (row i of a wave = one word of 64 consecutive right
+// tuples), left keys come in by scalar loads, and nlj_match16's v_cmpx
+// compares find the lanes with an equal key.  Only a wave whose tile has
+// such a lane rescans that tile, one ballot per (left key, word): the ballot
+// is the word's match mask for that left tuple, and non-zero masks are
+// appended as (left index, word, mask) triples.
+__global__ __launch_bounds__(kNljThreads) void k_nlj_bitmap(
+    const uint32_t *__restrict__ lkeys, uint64_t nL, const sjb_tuple_t *__restrict__ R, uint64_t nR,
     uint64_t l_split, uint64_t *__restrict__ masks, uint64_t mask_cap,
     unsigned long long *__restrict__ n_masks) {
-    __shared__ uint32_t s_lk[kBmLTile];
-    const uint64_t r = uint64_t(blockIdx.x) * kBmThreads + threadIdx.x;
-    const uint64_t word = r / kWave;
-    const bool valid = r < nR;
-    const uint32_t rk = valid ? R[r].key : 0u;
+    const uint64_t rbase = uint64_t(blockIdx.x) * kNljRTile;
+    const uint64_t wbase = rbase + (threadIdx.x & ~uint32_t(kWave - 1));  // lane 0's right index, row 0
+    const bool leader = __lane_id() == 0;
+    uint32_t sk[kNljPer];
+    bool valid[kNljPer];
+#pragma unroll
+    for (int i = 0; i < kNljPer; i++) {
+        const uint64_t r = rbase + threadIdx.x + uint64_t(i) * kNljThreads;
+        valid[i] = r < nR;
+        sk[i] = valid[i] ? R[r].key : 0u;
+    }
     const uint64_t lbeg = uint64_t(blockIdx.y) * l_split;
     const uint64_t lend = min(nL, lbeg + l_split);
-    for (uint64_t t0 = lbeg; t0 < lend; t0 += kBmLTile) {
-        const int tn = int(min<uint64_t>(kBmLTile, lend - t0));
-        for (int i = threadIdx.x; i < tn; i += kBmThreads) s_lk[i] = L[t0 + i].key;
-        __syncthreads();
+    for (uint64_t t0 = lbeg; t0 < lend; t0 += kNljLTile) {
+        const int tn = int(min<uint64_t>(kNljLTile, lend - t0));
+        const uint32_t *__restrict__ tk = lkeys + t0;
+        const int t16 = tn & ~31;
+        // the count kernel's prefilter (reads past t16 stay in the slack)
+        uint32_t kc[16], kn[16];
+        uint64_t hit = 0;
+#pragma unroll
+        for (int c = 0; c < 16; c++) kc[c] = tk[c];
+        for (int e = 0; e < t16; e += 32) {
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): kc has landed
+#pragma unroll
+            for (int c = 0; c < 16; c++) kn[c] = tk[e + 16 + c];
+            hit |= nlj_match16(kc, sk);
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // kn has landed
+#pragma unroll
+            for (int c = 0; c < 16; c++) kc[c] = tk[e + 32 + c];
+            hit |= nlj_match16(kn, sk);
+        }
+        uint32_t m = 0xFFFFFFFFu;
+        for (int e = t16; e < tn; e++) {
+            const uint32_t lk = tk[e];
+#pragma unroll
+            for (int i = 0; i < kNljPer; i++) m = min(m, lk ^ sk[i]);
+        }
+        if ((hit | __ballot(m == 0)) == 0) continue;  // wave-uniform
         for (int e = 0; e < tn; e++) {
-            const uint64_t mask = __ballot(valid && s_lk[e] == rk);
-            if (mask != 0 && (threadIdx.x & (kWave - 1)) == 0) {
-                const unsigned long long slot = atomicAdd(n_masks, 1ull);
-                if (slot < mask_cap) {
-                    masks[3 * slot] = t0 + e;
-                    masks[3 * slot + 1] = word;
-                    masks[3 * slot + 2] = mask;
+            const uint32_t lk = tk[e];
+#pragma unroll
+            for (int i = 0; i < kNljPer; i++) {
+                const uint64_t mask = __ballot(valid[i] && lk == sk[i]);
+                if (mask != 0 && leader) {
+                    const unsigned long long slot = atomicAdd(n_masks, 1ull);
+                    if (slot < mask_cap) {
+                        masks[3 * slot] = t0 + uint64_t(e);
+                        masks[3 * slot + 1] = (wbase + uint64_t(i) * kNljThreads) / kWave;
+                        masks[3 * slot + 2] = mask;
+                    }
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -395,15 +447,10 @@ int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_t
                       uint64_t nR, uint32_t flags, sjb_result_tuple_t *out, uint64_t out_cap,
                       sjb_window_result_t *d_out, uint64_t *n_masks_host) {
     (void)flags;
-    const uint64_t gx = (nR + kBmThreads - 1) / kBmThreads;
-    uint64_t gy = (uint64_t(c->n_cu) * 4 + gx - 1) / gx;
-    const uint64_t max_gy = (nL + kBmLTile - 1) / kBmLTile;
-    if (gy > max_gy) gy = max_gy;
-    if (gy < 1) gy = 1;
-    if (gx > 65535) return SJB_E_CAPACITY;
-    uint64_t l_split = (nL + gy - 1) / gy;
-    l_split = (l_split + kBmLTile - 1) / kBmLTile * kBmLTile;
-    gy = (nL + l_split - 1) / l_split;
+    dim3 grid;
+    uint64_t l_split;
+    int rc0 = nlj_prepare(c, L, nL, nR, nullptr, &grid, &l_split);
+    if (rc0) return rc0;
     auto *cnt = reinterpret_cast<unsigned long long *>(c->d_cursor) + 1;
     auto *cur = reinterpret_cast<unsigned long long *>(c->d_cursor);
     // The number of (left tuple, 64-key word) masks with a match is a
@@ -414,8 +461,8 @@ int launch_nlj_bitmap(sjb_ctx *c, const sjb_tuple_t *L, uint64_t nL, const sjb_t
     for (int pass = 0;; pass++) {
         int rc = launch_fill_u64(c->stream, c->d_cursor, 0, 2);
         if (rc) return rc;
-        k_nlj_bitmap<<<dim3(unsigned(gx), unsigned(gy)), kBmThreads, 0, c->stream>>>(
-            L, nL, R, nR, l_split, c->d_masks, c->mask_cap, cnt);
+        k_nlj_bitmap<<<grid, kNljThreads, 0, c->stream>>>(c->d_nlj_keys, nL, R, nR, l_split, c->d_masks,
+                                                          c->mask_cap, cnt);
         SJB_HIP_TRY(hipGetLastError());
         SJB_HIP_TRY(hipMemcpyAsync(&nm, cnt, sizeof(nm), hipMemcpyDeviceToHost, c->stream));
         SJB_HIP_TRY(hipStreamSynchronize(c->stream));

@@ -93,6 +93,49 @@ def test_materialised_pairs(ctx, torch_mod, orc, alg, method):
     assert np.array_equal(sort_rows(rows), ref_rows)
 
 
+@pytest.mark.parametrize("key_range", [16, 1000, 1 << 24])
+@pytest.mark.parametrize("size", [(1, 1), (100, 37), (1023, 1025), (5000, 4096), (16389, 20011)])
+def test_bitmap_pairs(ctx, torch_mod, orc, key_range, size):
+    """NLJ Bitmap at ragged sizes (partial right rows and left tiles, one
+    and several tiles per workgroup) and dense to sparse matches: every
+    (left, 64-key word) mask is found once, so rows and checksums equal the
+    pair enumeration's."""
+    nL, nR = size
+    if key_range == 16 and nL * nR > 1 << 23:
+        nL, nR = 1501, 1499  # keep the dense case's pair list small
+    print(f"case bitmap range={key_range} size={(nL, nR)}", flush=True)
+    R, S = make(nL, key_range, 3000 + nL), make(nR, key_range, 4000 + nR)
+    ref_rows = sort_rows(np_pair_rows(R, S))
+    o, _ = orc.lazy_hj(R, S, 1 << 16)
+    cap = len(ref_rows) + 10
+    out = torch_mod.zeros(cap * 32, dtype=torch_mod.uint8, device="cuda")
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    st = ctx.lazy_join(NLJ, BM, 0, dR, nL, dS, nR, out, cap)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    assert st.pairs_written == len(ref_rows)
+    rows = out.cpu().numpy().view(ref_rows.dtype)[: st.pairs_written]
+    assert np.array_equal(sort_rows(rows), ref_rows)
+
+
+def test_bitmap_window16(ctx, torch_mod, orc, sjbmod):
+    """NLJ Bitmap on a full 2^16 x 2^16 window (keys uniform in [0, 2^24),
+    ~256 pairs): rows and checksums against the pair enumeration and the
+    oracle, and the count equals the count kernel's."""
+    W = 1 << 16
+    R = sjbmod.generate(W, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=777)
+    S = sjbmod.generate(W, sjbmod.DIST_UNIFORM, 1 << 24, 0.0, seed=778)
+    ref_rows = sort_rows(np_pair_rows(R, S))
+    o, _ = orc.lazy_hj(R, S, 1 << 16)
+    cap = len(ref_rows) + 10
+    out = torch_mod.zeros(cap * 32, dtype=torch_mod.uint8, device="cuda")
+    dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+    st = ctx.lazy_join(NLJ, BM, 0, dR, W, dS, W, out, cap)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    rows = out.cpu().numpy().view(ref_rows.dtype)[: st.pairs_written]
+    assert np.array_equal(sort_rows(rows), ref_rows)
+    assert run_count(ctx, torch_mod, NLJ, R, S).result_count == o.count
+
+
 @pytest.mark.parametrize("spread", [1 << 12, (1 << 20) - 1, 1 << 21, 1 << 32])
 @pytest.mark.parametrize("size", [(1000, 900), (3000, 2500), (5000, 4096)])
 def test_smj_sort_key_spans(ctx, torch_mod, orc, spread, size):
