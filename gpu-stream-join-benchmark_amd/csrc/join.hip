@@ -194,7 +194,7 @@ __device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_
         b = o[p + 1];
     }
     uint32_t wt;
-    const uint32_t ex = wave_excl_scan(b - a, &wt);
+    const uint32_t ex = wave_excl_scan_full(b - a, &wt);
     const int wid = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) ws[wid] = wt;
     __syncthreads();
@@ -1011,11 +1011,7 @@ __device__ __forceinline__ TileSort gather_tile(uint64_t *s, uint64_t *tmp, uint
             for (int q = 0; q < int(kSmjDirect) / kJoinThreads; q++)
                 cnt[q * kJoinThreads + threadIdx.x] = 0;
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            mn = min(mn, uint32_t(__shfl_xor(int(mn), off)));
-            mx = max(mx, uint32_t(__shfl_xor(int(mx), off)));
-        }
+        wave_minmax_full(mn, mx);
         constexpr int kW = kJoinThreads / kWave;
         if ((threadIdx.x & (kWave - 1)) == 0) {
             s_mm[threadIdx.x / kWave] = mn;
@@ -1321,9 +1317,9 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED, true>(io, v, bshift, s_mem, s_ws, &s_base, acc);
     // block sum with a single barrier (nothing runs after it)
-    acc.count = wave_sum(acc.count);
-    acc.c1 = wave_sum(acc.c1);
-    acc.c2 = wave_sum(acc.c2);
+    acc.count = wave_sum_full(acc.count);
+    acc.c1 = wave_sum_full(acc.c1);
+    acc.c2 = wave_sum_full(acc.c2);
     constexpr int kW = kJoinThreads / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const int w = threadIdx.x / kWave;

@@ -193,11 +193,7 @@ __global__ __launch_bounds__(kMinmaxThreads) void k_minmax_partial(const sjb_tup
         mn = min(mn, k);
         mx = max(mx, k);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, kWave));
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, kWave));
-    }
+    wave_minmax_full(mn, mx);
     const int wid = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) {
         smin[wid] = mn;
@@ -223,11 +219,7 @@ __global__ __launch_bounds__(kMinmaxThreads) void k_minmax_final(const uint32_t 
         mn = min(mn, partial[2 * b]);
         mx = max(mx, partial[2 * b + 1]);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, kWave));
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, kWave));
-    }
+    wave_minmax_full(mn, mx);
     const int wid = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) {
         smin[wid] = mn;
@@ -355,8 +347,8 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
         heavy_drain(*hs, hs->q[threadIdx.x / kWave], qn, left, hcnt, hsum);
 #pragma unroll
         for (int i = 0; i < kHeavyReg; i++) {
-            const uint64_t c = wave_sum(uint64_t(hcnt[i]));
-            const uint64_t v = wave_sum(hsum[i]);
+            const uint64_t c = wave_sum_full(uint64_t(hcnt[i]));
+            const uint64_t v = wave_sum_full(hsum[i]);
             if ((threadIdx.x & (kWave - 1)) == 0 && c) {
                 atomicAdd(&hs->cnt[i], uint32_t(c));
                 atomicAdd(reinterpret_cast<unsigned long long *>(&hs->sum[i]), (unsigned long long)v);
@@ -630,11 +622,7 @@ __device__ __forceinline__ void sample_range(const sjb_tuple_t *__restrict__ L, 
             mx = k;
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, kWave));
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, kWave));
-    }
+    wave_minmax_full(mn, mx);
     if ((threadIdx.x & (kWave - 1)) == 0 && t < 2 * kS) {
         atomicMin(&s_mm[0], mn);
         atomicMax(&s_mm[1], mx);

@@ -1,8 +1,11 @@
 // sjb_device.hpp — device-side primitives shared by every gfx950 kernel.
 //
 // Checksum formulas are the ones documented in include/sjb.h.  Reductions
-// are written for 64-lane wavefronts (CDNA4): a wave reduces with
-// __shfl_xor over 6 steps, then one LDS slot per wave finishes the block.
+// are written for 64-lane wavefronts (CDNA4).  Where the whole wave is
+// active (next to a barrier) a wave reduces and scans with DPP lane moves
+// and readlane — VALU instructions of a few cycles' latency — instead of
+// __shfl, which goes through the LDS unit (ds_bpermute) at LDS latency per
+// step; then one LDS slot per wave finishes the block.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,10 +49,70 @@ struct Acc {
     }
 };
 
+// Sum over the active lanes of the wave (any exec mask).
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
+}
+
+// DPP move of lane values within 16-lane rows (VALU).  Lanes whose source
+// lies outside the row, and rows outside ROW_MASK, get 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, ROW_MASK, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    return uint64_t(dpp_u32<CTRL>(uint32_t(v))) | (uint64_t(dpp_u32<CTRL>(uint32_t(v >> 32))) << 32);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+    return uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), lane))) |
+           (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), lane))) << 32);
+}
+
+// Sum over the 64 lanes, returned in every lane.  Every lane must be active:
+// DPP and readlane read an inactive lane's register as it stands.
+__device__ __forceinline__ uint64_t wave_sum_full(uint64_t v) {
+    v += dpp_u64<0xB1>(v);   // quad_perm [1,0,3,2]: + lane ^ 1
+    v += dpp_u64<0x4E>(v);   // quad_perm [2,3,0,1]: + lane ^ 2 -> quad sums
+    v += dpp_u64<0x141>(v);  // row_half_mirror: + the other quad of the 8 lanes
+    v += dpp_u64<0x140>(v);  // row_mirror: + the other 8 lanes -> row sums
+    return readlane_u64(v, 0) + readlane_u64(v, 16) + readlane_u64(v, 32) + readlane_u64(v, 48);
+}
+
+// Minimum and maximum over the 64 lanes, returned in every lane.  Every lane
+// must be active (as for wave_sum_full); the four lane moves have a source
+// in every lane.
+__device__ __forceinline__ void wave_minmax_full(uint32_t &mn, uint32_t &mx) {
+    mn = min(mn, dpp_u32<0xB1>(mn));
+    mx = max(mx, dpp_u32<0xB1>(mx));
+    mn = min(mn, dpp_u32<0x4E>(mn));
+    mx = max(mx, dpp_u32<0x4E>(mx));
+    mn = min(mn, dpp_u32<0x141>(mn));
+    mx = max(mx, dpp_u32<0x141>(mx));
+    mn = min(mn, dpp_u32<0x140>(mn));
+    mx = max(mx, dpp_u32<0x140>(mx));
+    const uint32_t n0 = min(uint32_t(__builtin_amdgcn_readlane(int(mn), 0)), uint32_t(__builtin_amdgcn_readlane(int(mn), 16)));
+    const uint32_t n1 = min(uint32_t(__builtin_amdgcn_readlane(int(mn), 32)), uint32_t(__builtin_amdgcn_readlane(int(mn), 48)));
+    const uint32_t x0 = max(uint32_t(__builtin_amdgcn_readlane(int(mx), 0)), uint32_t(__builtin_amdgcn_readlane(int(mx), 16)));
+    const uint32_t x1 = max(uint32_t(__builtin_amdgcn_readlane(int(mx), 32)), uint32_t(__builtin_amdgcn_readlane(int(mx), 48)));
+    mn = min(n0, n1);
+    mx = max(x0, x1);
+}
+
+// Exclusive prefix over the 64 lanes; *total = the wave's sum.  Every lane
+// must be active (as for wave_sum_full).
+__device__ __forceinline__ uint32_t wave_excl_scan_full(uint32_t v, uint32_t *total) {
+    uint32_t x = v;
+    x += dpp_u32<0x111>(x);       // row_shr:1
+    x += dpp_u32<0x112>(x);       // row_shr:2
+    x += dpp_u32<0x114>(x);       // row_shr:4
+    x += dpp_u32<0x118>(x);       // row_shr:8 -> inclusive scan of each 16-lane row
+    x += dpp_u32<0x142, 0xA>(x);  // row_bcast:15: rows 1 and 3 add the previous row's last lane
+    x += dpp_u32<0x143, 0xC>(x);  // row_bcast:31: rows 2 and 3 add lane 31 (rows 0-1)
+    *total = uint32_t(__builtin_amdgcn_readlane(int(x), kWave - 1));
+    return x - v;
 }
 
 // Block-wide sum of an Acc; result valid in thread 0.  `scratch` holds
@@ -58,9 +121,9 @@ __device__ __forceinline__ Acc block_sum(Acc a, uint64_t *scratch) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
-    a.count = wave_sum(a.count);
-    a.c1 = wave_sum(a.c1);
-    a.c2 = wave_sum(a.c2);
+    a.count = wave_sum_full(a.count);
+    a.c1 = wave_sum_full(a.c1);
+    a.c2 = wave_sum_full(a.c2);
     if (lane == 0) {
         scratch[wid] = a.count;
         scratch[nw + wid] = a.c1;
@@ -98,7 +161,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total) 
 // block must call it (two barriers).
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *ws, uint32_t *total) {
     uint32_t wt;
-    const uint32_t ex = wave_excl_scan(v, &wt);
+    const uint32_t ex = wave_excl_scan_full(v, &wt);
     const int wid = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) ws[wid] = wt;
     __syncthreads();

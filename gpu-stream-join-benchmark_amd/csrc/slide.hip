@@ -224,11 +224,11 @@ __global__ __launch_bounds__(kSlideThreads) void k_slide_step(
         atomicAdd(reinterpret_cast<unsigned long long *>(&own[g].sum), (unsigned long long)(sb * sign));
     }
     // new keys counted once per wave instead of one global atomic each
-    const uint64_t fw = wave_sum(uint64_t(fresh_n));
+    const uint64_t fw = wave_sum_full(uint64_t(fresh_n));
     if ((threadIdx.x & (kWave - 1)) == 0 && fw) atomicAdd(used, uint32_t(fw));
     if (PROBE) {
-        cnt = wave_sum(cnt * sign);
-        c1 = wave_sum(c1 * sign);
+        cnt = wave_sum_full(cnt * sign);
+        c1 = wave_sum_full(c1 * sign);
         if ((threadIdx.x & (kWave - 1)) == 0) {
             s_red[threadIdx.x / kWave] = cnt;
             s_red[kSlideThreads / kWave + threadIdx.x / kWave] = c1;
