@@ -1079,6 +1079,68 @@ struct SmjLds {
     uint64_t pa[GROUPED ? kSmjCap + 1 : 1];       // grouped: prefix sums of A over the sorted build tile
 };
 
+// Count / C1 / C2 of the un probe records in m.p against a build tile that
+// the counting sort ordered (m.dir: first rank of each key offset from
+// kmin).  Every lookup is done first, kept as (first rank | count << 16) per
+// probe slot; after a barrier the key directory holds the waves' match
+// queues, and the first two matches of every probe go through the queue so
+// the checksum math runs on full waves (later matches, rare under uniform
+// keys, are summed in place).  The directory is gone afterwards: for the
+// last probe tile of a build tile.
+template <bool C2>
+__device__ __forceinline__ void smj_probe_queued(SmjLds<false> &m, int un, uint32_t kmin, Acc &acc) {
+    constexpr int kSlots = kSmjCap / kJoinThreads;
+    static_assert(sizeof(m.dir) >= (kJoinThreads / kWave) * sizeof(PairQ), "one queue per wave in the directory");
+    static_assert(offsetof(SmjLds<false>, dir) % alignof(PairQ) == 0, "queues aligned in the directory");
+    uint32_t lc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) {
+        const int j = k * kJoinThreads + int(threadIdx.x);
+        lc[k] = 0;
+        if (j < un) {
+            const uint32_t d = uint32_t(m.p[j] >> 32) - kmin;
+            if (d < kSmjDirect) {
+                const uint32_t lo = m.dir[d], hi = m.dir[d + 1];
+                lc[k] = lo | ((hi - lo) << 16);
+            }
+        }
+    }
+    __syncthreads();  // every lookup done: the directory becomes the queues
+    PairQ &q = reinterpret_cast<PairQ *>(m.dir)[threadIdx.x / kWave];
+    uint32_t qn = 0;  // queued matches (wave-uniform)
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) {
+        const int j = k * kJoinThreads + int(threadIdx.x);
+        const uint32_t lo = lc[k] & 0xFFFFu, cnt = lc[k] >> 16;
+        uint32_t key = 0, sv = 0;
+        if (cnt) {
+            const uint64_t r = m.p[j];
+            key = uint32_t(r >> 32);
+            sv = uint32_t(r);
+        }
+        acc.count += cnt;
+        for (uint32_t e = 2; e < cnt; e++) pair_acc<C2>(key, uint32_t(m.b[lo + e]), sv, acc);
+        for (uint32_t e = 0; e < 2; e++) {
+            const bool has = cnt > e;
+            const uint64_t mask = __ballot(has);
+            if (mask == 0) break;  // wave-uniform
+            const uint32_t c = uint32_t(__popcll(mask));
+            if (qn + c > uint32_t(kWave)) {
+                pairq_drain<C2>(q, qn, acc);
+                qn = 0;
+            }
+            if (has) {
+                const uint32_t pos = qn + lanes_below(mask);
+                q.key[pos] = key;
+                q.sv[pos] = sv;
+                q.rv[pos] = uint32_t(m.b[lo + e]);
+            }
+            qn += c;
+        }
+    }
+    pairq_drain<C2>(q, qn, acc);
+}
+
 // Joins one partition (SMJ): each build tile is bitonic-sorted in LDS and
 // every probe element is merged into it by co-ranking its key (lower /
 // upper bound: the one-sided merge path), so the probe tile itself needs
@@ -1184,7 +1246,14 @@ __device__ __forceinline__ void smj_join_part(const JoinIO &io, const PartView &
                     }
                 }
             } else if (!WRITE) {
-                for (int j = threadIdx.x; j < un; j += kJoinThreads) {
+                bool queued = false;
+                if constexpr (PER_WINDOW && !GROUPED) {
+                    if (ts.direct && u0 + uint32_t(kSmjCap) >= np) {  // block-uniform
+                        smj_probe_queued<C2>(m, un, ts.kmin, acc);
+                        queued = true;
+                    }
+                }
+                for (int j = threadIdx.x; !queued && j < un; j += kJoinThreads) {
                     const uint32_t k = uint32_t(s_p[j] >> 32), sv = uint32_t(s_p[j]);
                     int lo, hi;
                     if (PER_WINDOW && ts.direct) {  // the counting sort's key directory
