@@ -137,11 +137,12 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.kind = part_kind(alg, learned);
     p.payload = write ? PAY_IDX : PAY_VAL;
     p.peel = learned && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
-    p.ncL = int((nL + kChunk - 1) / kChunk);
-    p.ncR = int((nR + kChunk - 1) / kChunk);
+    p.chunk = c->small_chunks && nL + nR <= kSmallWindow ? kSmallChunk : kChunk;
+    p.ncL = int((nL + p.chunk - 1) / p.chunk);
+    p.ncR = int((nR + p.chunk - 1) / p.chunk);
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
-    // the count paths that k_partition_pipe partitions sum in the join
-    p.direct = SJB_PART_PIPE && direct && !write && !p.peel && p.kind != PART_LEARNED ? direct : nullptr;
+    // count paths sum in the join (the partition pass zeroes the result)
+    p.direct = direct && !write && !p.peel && p.kind != PART_LEARNED ? direct : nullptr;
     int rc;
     if (p.kind == PART_RANGE) {
         rc = launch_minmax(c, p.ps, L, nL, R, nR, p.logF);
@@ -406,7 +407,10 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
         if (need > c->opts.hbm_budget_bytes) return fail(SJB_E_BUDGET);
     }
     const uint64_t nchunks = std::min<uint64_t>((cap + kChunk - 1) / kChunk, kMaxChunks);
-    c->offs_cap = ((uint64_t(1) << kMaxLogF) + 1) * 2 * std::max<uint64_t>(nchunks, 1);
+    // offsets rows: two sides of kChunk chunks, or a small window's
+    // kSmallChunk chunks (both sides together at most kSmallWindow tuples)
+    const uint64_t small_rows = std::min<uint64_t>(2 * cap, kSmallWindow) / kSmallChunk + 2;
+    c->offs_cap = ((uint64_t(1) << kMaxLogF) + 1) * std::max<uint64_t>(2 * std::max<uint64_t>(nchunks, 1), small_rows);
     const uint64_t n_partials = std::max<uint64_t>(65536, 1ull << kMaxLogF);
     c->mask_cap = std::max<uint64_t>(cap, 1ull << 20);
     if ((rc = dev_alloc(c, (void **)&c->d_stage[0], cap * sizeof(sjb_tuple_t))) ||

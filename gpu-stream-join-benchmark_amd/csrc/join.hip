@@ -1368,7 +1368,7 @@ template <int ALG, bool C2, bool WRITE, bool GROUPED>
 __global__ __launch_bounds__(kJoinThreads)
 __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 : 6))) void k_join(
     JoinIO io, const uint32_t *__restrict__ offs,
-                                                       int logF, int ncL, int ncR, uint32_t nL,
+                                                       int logF, int ncL, int ncR, uint32_t nL, int chunk,
                                                        int bshift, uint64_t *__restrict__ partials,
                                                        sjb_window_result_t *__restrict__ direct) {
     __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, GROUPED, WRITE>::value];
@@ -1380,8 +1380,8 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
     const int p = xcd_partition(blockIdx.x, 1 << logF);
     const PartView v =
         ncL + ncR <= kJoinThreads && ncL % kWave == 0
-            ? load_view_small(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws)
-            : load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, kChunk, idcap_of<ALG, GROUPED>(), s_ws);
+            ? load_view_small(dyn, offs, logF, p, ncL, ncR, 0u, nL, uint32_t(chunk), idcap_of<ALG, GROUPED>(), s_ws)
+            : load_view(dyn, offs, logF, p, ncL, ncR, 0u, nL, uint32_t(chunk), idcap_of<ALG, GROUPED>(), s_ws);
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED, true>(io, v, bshift, s_mem, s_ws, &s_base, acc);
@@ -1560,7 +1560,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
 #define SJB_J(C2_, W_, G_)                                                                          \
     k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL,      \
-                                                                 p.ncR, uint32_t(p.nL), bshift,    \
+                                                                 p.ncR, uint32_t(p.nL), p.chunk, bshift, \
                                                                  c->d_partials, p.direct)
     if (p.payload == PAY_IDX) {
         if (c2)

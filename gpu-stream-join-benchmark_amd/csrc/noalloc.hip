@@ -106,6 +106,7 @@ extern "C" int sjb_lazy_join_noalloc(int algorithm, uint32_t flags, const sjb_tu
     c.n_cu = d->n_cu;
     c.cap = kNaMax;
     c.offs_cap = kNaOffsWords;
+    c.small_chunks = false;  // the offsets slice holds kChunk chunks only
     c.d_part = reinterpret_cast<Rec *>(d->ptr + kOffPart);
     c.d_offs = reinterpret_cast<uint32_t *>(d->ptr + kOffOffs);
     c.d_partials = reinterpret_cast<uint64_t *>(d->ptr + kOffPartials);

@@ -393,23 +393,27 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
     }
 }
 
-// One window: blocks [0, ncL) take left chunks and [ncL, ncL+ncR) right
-// chunks; chunk b's offsets row is offs[b * (F+1)].
-template <int KIND, int PAY, bool PEEL>
+// One window: blocks [0, ncL) take left chunks of CH tuples and
+// [ncL, ncL+ncR) right chunks; chunk b's offsets row is offs[b * (F+1)].
+// Block 0 zeroes the window result the join adds into (direct non-null).
+template <int KIND, int PAY, bool PEEL, int CH>
 __global__ __launch_bounds__(kPartThreads) void k_partition(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
     uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, uint64_t *__restrict__ heavy_acc) {
+    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, uint64_t *__restrict__ heavy_acc,
+    sjb_window_result_t *__restrict__ direct) {
     extern __shared__ uint32_t lds[];
     __shared__ HeavySet hs[1];
+    if (direct && blockIdx.x == 0 && threadIdx.x < sizeof(sjb_window_result_t) / 8)
+        reinterpret_cast<uint64_t *>(direct)[threadIdx.x] = 0;
     PartFn<KIND> part;
     part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
     const bool left = int(blockIdx.x) < ncL;
     const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
     const uint64_t n = left ? nL : nR;
-    const uint64_t base = uint64_t(chunk) * kChunk;
-    const int len = int(min<uint64_t>(kChunk, n - base));
-    partition_chunk<KIND, PAY, kChunk, PEEL>(left ? L : R, base, len, logF, part,
+    const uint64_t base = uint64_t(chunk) * CH;
+    const int len = int(min<uint64_t>(CH, n - base));
+    partition_chunk<KIND, PAY, CH, PEEL>(left ? L : R, base, len, logF, part,
                                              offs + uint64_t(blockIdx.x) * ((1 << logF) + 1),
                                              part_out + (left ? 0 : nL) + base, lds, hs, rmi, left,
                                              heavy_acc);
@@ -733,9 +737,25 @@ size_t partition_lds_bytes_chunk(int logF, int chunk) {
 }
 size_t partition_lds_bytes(int logF) { return partition_lds_bytes_chunk(logF, kChunk); }
 
+template <int KIND, bool PEEL, int CH>
+static int partition_chunks(const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R, sjb_ctx *c) {
+    const size_t lds = partition_lds_bytes_chunk(p.logF, CH);
+    const int nblk = p.ncL + p.ncR;
+    if (p.payload == PAY_VAL)
+        k_partition<KIND, PAY_VAL, PEEL, CH><<<nblk, kPartThreads, lds, p.ps>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy, p.direct);
+    else
+        k_partition<KIND, PAY_IDX, false, CH><<<nblk, kPartThreads, lds, p.ps>>>(
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy, p.direct);
+    SJB_HIP_TRY(hipGetLastError());
+    return SJB_OK;
+}
+
 template <int KIND, bool PEEL>
 static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
                           const sjb_tuple_t *R) {
+    if (p.chunk == kSmallChunk) return partition_chunks<KIND, PEEL, kSmallChunk>(p, L, R, c);
+    if (p.chunk != kChunk) return SJB_E_INVALID;
     const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;
 #if SJB_PART_PIPE
@@ -749,14 +769,9 @@ static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
         }
     }
 #endif
-    if (p.payload == PAY_VAL)
-        k_partition<KIND, PAY_VAL, PEEL><<<nblk, kPartThreads, lds, p.ps>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy);
-    else
-        k_partition<KIND, PAY_IDX, false><<<nblk, kPartThreads, lds, p.ps>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy);
-    SJB_HIP_TRY(hipGetLastError());
-    return SJB_OK;
+    (void)lds;
+    (void)nblk;
+    return partition_chunks<KIND, PEEL, kChunk>(p, L, R, c);
 }
 
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R) {
@@ -857,16 +872,16 @@ int partition_setup(sjb_ctx *c) {
     const int need = int(partition_lds_bytes(kMaxLogF));
     (void)c;
     const void *fns[] = {
-        (const void *)k_partition<PART_HASH, PAY_VAL, false>,
-        (const void *)k_partition<PART_HASH, PAY_IDX, false>,
-        (const void *)k_partition<PART_HASH, PAY_VAL, true>,
+        (const void *)k_partition<PART_HASH, PAY_VAL, false, kChunk>,
+        (const void *)k_partition<PART_HASH, PAY_IDX, false, kChunk>,
+        (const void *)k_partition<PART_HASH, PAY_VAL, true, kChunk>,
         (const void *)k_partition_batch<PART_HASH, kChunk, true>,
         (const void *)k_partition_batch<PART_HASH, 4096, true>,
-        (const void *)k_partition<PART_RANGE, PAY_VAL, false>,
-        (const void *)k_partition<PART_RANGE, PAY_IDX, false>,
-        (const void *)k_partition<PART_LEARNED, PAY_VAL, false>,
-        (const void *)k_partition<PART_LEARNED, PAY_IDX, false>,
-        (const void *)k_partition<PART_LEARNED, PAY_VAL, true>,
+        (const void *)k_partition<PART_RANGE, PAY_VAL, false, kChunk>,
+        (const void *)k_partition<PART_RANGE, PAY_IDX, false, kChunk>,
+        (const void *)k_partition<PART_LEARNED, PAY_VAL, false, kChunk>,
+        (const void *)k_partition<PART_LEARNED, PAY_IDX, false, kChunk>,
+        (const void *)k_partition<PART_LEARNED, PAY_VAL, true, kChunk>,
         (const void *)k_partition_batch<PART_HASH, kChunk, false>,
         (const void *)k_partition_batch<PART_RANGE, kChunk, false>,
         (const void *)k_partition_batch<PART_RANGE, kChunk, true>,
@@ -877,6 +892,10 @@ int partition_setup(sjb_ctx *c) {
         (const void *)k_partition_batch<PART_RANGE, 4096, false>,
         (const void *)k_partition_batch<PART_LEARNED, 4096, false>,
         (const void *)k_partition_batch<PART_LEARNED, 4096, true>,
+        (const void *)k_partition<PART_HASH, PAY_VAL, false, kSmallChunk>,
+        (const void *)k_partition<PART_HASH, PAY_IDX, false, kSmallChunk>,
+        (const void *)k_partition<PART_RANGE, PAY_VAL, false, kSmallChunk>,
+        (const void *)k_partition<PART_RANGE, PAY_IDX, false, kSmallChunk>,
         (const void *)k_partition_pipe<PART_HASH, PAY_VAL>,
         (const void *)k_partition_pipe<PART_RANGE, PAY_VAL>,
     };

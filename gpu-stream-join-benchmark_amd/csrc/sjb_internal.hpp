@@ -46,7 +46,18 @@ typedef uint2 Rec;
 #ifndef SJB_SMJ_CAP
 #define SJB_SMJ_CAP 2048
 #endif
+#ifndef SJB_SMALL_CHUNK
+#define SJB_SMALL_CHUNK 2048
+#endif
+#ifndef SJB_SMALL_WINDOW
+#define SJB_SMALL_WINDOW (1 << 20)
+#endif
 constexpr int kChunk = SJB_CHUNK;              // tuples per partition-pass workgroup
+// Windows of at most kSmallWindow tuples (both sides) are partitioned in
+// kSmallChunk-tuple chunks, one workgroup each: in kChunk chunks a 2^17-
+// tuple window would keep 8 of the chip's workgroups busy.
+constexpr int kSmallChunk = SJB_SMALL_CHUNK;
+constexpr uint64_t kSmallWindow = SJB_SMALL_WINDOW;
 constexpr int kPartThreads = SJB_PART_THREADS; // threads of the partition pass
 constexpr int kMaxLogF = 12;                   // fanout bound: 4096 partitions
 constexpr int kMaxChunks = 1024;               // chunks per side
@@ -155,6 +166,7 @@ struct WindowPlan {
     int payload;   // Payload
     uint32_t flags;
     uint64_t nL, nR;
+    int chunk;     // tuples per partition chunk: kChunk or kSmallChunk
     int ncL, ncR;  // chunks per side
     bool peel;     // learned + grouped: heavy keys aggregated apart
     // where the partition pass runs and writes (the ctx's own buffers and
@@ -165,7 +177,7 @@ struct WindowPlan {
     uint64_t *heavy;
     // non-null: the partition pass zeroes this result and every join
     // workgroup adds its (count, C1, C2) to it with 64-bit atomics, so no
-    // reduction kernel runs (count paths partitioned by k_partition_pipe)
+    // reduction kernel runs (count paths without heavy-key peeling)
     sjb_window_result_t *direct;
 };
 
@@ -176,6 +188,7 @@ struct sjb_ctx {
     hipStream_t stream = nullptr;
     sjb_options_t opts{};
     int n_cu = 0;
+    bool small_chunks = true;         // small windows in kSmallChunk chunks (offs_cap sized for them)
     uint64_t bytes = 0;
     uint64_t ext_bytes = 0;           // device bytes of live batch plans and sliding windows
     std::vector<void *> dev_allocs;
