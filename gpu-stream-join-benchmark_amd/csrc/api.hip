@@ -132,12 +132,13 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.flags = flags;
     p.nL = nL;
     p.nR = nR;
-    p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2));
+    const bool small = c->small_chunks && nL + nR <= kSmallWindow;
+    p.logF = choose_logF(alg, flags, nL, int(c->opts.fanout_log2), small);
     const bool learned = (flags & SJB_F_LEARNED) && c->rmi_valid;
     p.kind = part_kind(alg, learned);
     p.payload = write ? PAY_IDX : PAY_VAL;
     p.peel = learned && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
-    p.chunk = c->small_chunks && nL + nR <= kSmallWindow ? kSmallChunk : kChunk;
+    p.chunk = small ? kSmallChunk : kChunk;
     p.ncL = int((nL + p.chunk - 1) / p.chunk);
     p.ncR = int((nR + p.chunk - 1) / p.chunk);
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;

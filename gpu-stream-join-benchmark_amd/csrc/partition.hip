@@ -719,14 +719,22 @@ int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias) {
     return std::min(std::max(logF, 0), kMaxLogF);
 }
 
-// Fanout of a lone window: the tile rule above, raised to at least 2^9
-// partitions (two join workgroups per CU) while partitions keep >= 64 build
-// records — a small window, or one with few distinct keys and many matches,
-// would otherwise run its join on a few dozen workgroups.
-int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested) {
+// Fanout of a lone window: the tile rule above, raised to a floor while
+// partitions keep >= 64 build records — a small window, or one with few
+// distinct keys and many matches, would otherwise run its join on a few
+// dozen workgroups.  Windows in kSmallChunk chunks (small): ~1024 build
+// records per partition and at least 2^7 partitions, at most 2^9 (measured
+// best at 2^14..2^19 tuples per side: 2^7 up to 2^16, 2^8 at 2^18, 2^9 at
+// 2^19); windows in kChunk chunks: 2^9 partitions (two join workgroups per
+// CU).
+int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested, bool small) {
     if (requested > 0) return std::min(std::max(requested, 1), kMaxLogF);
     const int logF = std::max(batch_log_fanout(alg, flags, n_build, 0), 1);
     int floor_logF = 9;
+    if (small) {
+        floor_logF = 7;
+        while (floor_logF < 9 && (n_build >> (floor_logF + 1)) >= 1024) floor_logF++;
+    }
     while (floor_logF > logF && (n_build >> floor_logF) < 64) floor_logF--;
     return std::max(logF, floor_logF);
 }

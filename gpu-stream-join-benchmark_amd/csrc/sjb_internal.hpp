@@ -287,7 +287,7 @@ void dev_free(sjb_ctx *c, void **p, size_t bytes);
 int dev_regrow(sjb_ctx *c, void **p, size_t old_bytes, size_t bytes);
 
 // ---- launchers (partition.hip) ----
-int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested);
+int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested, bool small);
 int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
                   uint64_t nR, int logF);
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
