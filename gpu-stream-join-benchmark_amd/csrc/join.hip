@@ -262,8 +262,7 @@ __device__ __forceinline__ int xcd_partition(int bid, int F) {
 // bucket b.  Built by an LDS counting sort; a probe reads two bounds and
 // scans contiguous keys (independent loads, no pointer chasing).
 struct HjTable {
-    uint32_t key[kHjCap];
-    uint32_t pay[kHjCap];
+    uint2 kv[kHjCap];  // (key, payload): one 8-byte LDS read per entry
     uint32_t off[kHjNB + 1];
 };
 
@@ -312,8 +311,7 @@ __device__ __forceinline__ void hj_build(HjTable &t, const Rec (&r)[kHjPerT], ui
         const uint32_t i = k * kJoinThreads + threadIdx.x;
         if (i < tn) {
             const uint32_t pos = t.off[slot[k] >> 16] + (slot[k] & 0xFFFFu);
-            t.key[pos] = r[k].x;
-            t.pay[pos] = r[k].y;
+            t.kv[pos] = make_uint2(r[k].x, r[k].y);
         }
     }
     __syncthreads();
@@ -394,7 +392,7 @@ __device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[
                 end = t.off[b + 1];
             }
             for (;;) {
-                while (e < end && t.key[e] != s.x) e++;
+                while (e < end && t.kv[e].x != s.x) e++;
                 const bool has = e < end;
                 const uint64_t mask = __ballot(has);
                 if (mask == 0) break;  // wave-uniform
@@ -407,7 +405,7 @@ __device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[
                     const uint32_t pos = qn + lanes_below(mask);
                     q.key[pos] = s.x;
                     q.sv[pos] = s.y;
-                    q.rv[pos] = t.pay[e];
+                    q.rv[pos] = t.kv[e].y;
                     cnt++;
                     e++;
                 }
@@ -422,24 +420,35 @@ __device__ __forceinline__ void hj_probe_regs(const HjTable &t, const Rec (&rp)[
     for (int k = 0; k < kHjPerP; k++) {
         const uint32_t j = k * kJoinThreads + threadIdx.x;
         const Rec s = rp[k];
-        uint32_t m = 0, rv0 = 0, rv1 = 0;
+        uint32_t m = 0, rv0 = 0, rv1 = 0, lo = 0, hi = 0;
         if (j < un) {
             const uint32_t b = hj_bucket(s.x, bshift);
-            const uint32_t lo = t.off[b], hi = t.off[b + 1];
+            lo = t.off[b];
+            hi = t.off[b + 1];
+            // branch-free scan: selects keep the first two matches' payloads
             for (uint32_t e = lo; e < hi; e++) {
-                if (t.key[e] == s.x) {
-                    const uint32_t rv = t.pay[e];
-                    if (m == 0)
-                        rv0 = rv;
-                    else if (m == 1)
-                        rv1 = rv;
-                    else
-                        pair_acc<C2>(s.x, rv, s.y, acc);
-                    m++;
-                }
+                const uint2 kv = t.kv[e];
+                const bool eq = kv.x == s.x;
+                rv1 = eq && m == 1 ? kv.y : rv1;
+                rv0 = eq && m == 0 ? kv.y : rv0;
+                m += eq;
             }
         }
         acc.count += m;
+        // third and later matches (rare under uniform keys): the lanes that
+        // have them scan their bucket again and sum those pairs in place
+        if (__ballot(m > 2) != 0) {  // wave-uniform
+            if (m > 2) {
+                uint32_t seen = 0;
+                for (uint32_t e = lo; e < hi; e++) {
+                    const uint2 kv = t.kv[e];
+                    if (kv.x == s.x) {
+                        if (seen >= 2) pair_acc<C2>(s.x, kv.y, s.y, acc);
+                        seen++;
+                    }
+                }
+            }
+        }
         for (uint32_t r = 0; r < 2; r++) {
             const bool has = m > r;
             const uint64_t mask = __ballot(has);
@@ -478,7 +487,7 @@ __device__ __forceinline__ void hj_probe_write(const HjTable &t, const Rec *sp, 
         if (j < un) {
             s = sp[j];
             const uint32_t b = hj_bucket(s.x, bshift);
-            for (uint32_t e = t.off[b]; e < t.off[b + 1]; e++) m += t.key[e] == s.x;
+            for (uint32_t e = t.off[b]; e < t.off[b + 1]; e++) m += t.kv[e].x == s.x;
         }
         uint32_t tot;
         const uint32_t ex = block_excl_scan_u32(m, ws, &tot);
@@ -489,8 +498,8 @@ __device__ __forceinline__ void hj_probe_write(const HjTable &t, const Rec *sp, 
             const sjb_tuple_t st = R[s.y];
             const uint32_t b = hj_bucket(s.x, bshift);
             for (uint32_t e = t.off[b]; e < t.off[b + 1]; e++) {
-                if (t.key[e] != s.x) continue;
-                const sjb_tuple_t lt = L[t.pay[e]];
+                if (t.kv[e].x != s.x) continue;
+                const sjb_tuple_t lt = L[t.kv[e].y];
                 const uint64_t a = hash_a(lt.key, lt.val);
                 acc.count++;
                 acc.c1 += a * hash_b(st.key, st.val);
