@@ -798,12 +798,34 @@ __device__ __forceinline__ void agg_join_part(const JoinIO &io, const PartView &
 // ---------------------------------------------------------------------------
 // SMJ
 // ---------------------------------------------------------------------------
+// Value of lane (lane ^ m), m a power of two below 64 known at compile time
+// after unrolling; every lane of the wave active.  m = 1, 2: DPP quad_perm;
+// m = 4, 8: a row_shl:m move into the banks (4-lane groups of a 16-lane
+// row) whose partner lies above and a row_shr:m move into the others —
+// VALU instructions, no LDS round trip; m = 16: ds_swizzle (xor mask within
+// 32 lanes, no address computation); m = 32: ds_bpermute.
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
+    switch (m) {
+        case 1: return dpp_u32<0xB1>(v);  // quad_perm [1,0,3,2]
+        case 2: return dpp_u32<0x4E>(v);  // quad_perm [2,3,0,1]
+        case 4: {
+            const int up = __builtin_amdgcn_update_dpp(0, int(v), 0x104, 0xF, 0x5, false);  // row_shl:4, banks 0, 2
+            return uint32_t(__builtin_amdgcn_update_dpp(up, int(v), 0x114, 0xF, 0xA, false));  // row_shr:4, banks 1, 3
+        }
+        case 8: {
+            const int up = __builtin_amdgcn_update_dpp(0, int(v), 0x108, 0xF, 0x3, false);  // row_shl:8, banks 0, 1
+            return uint32_t(__builtin_amdgcn_update_dpp(up, int(v), 0x118, 0xF, 0xC, false));  // row_shr:8, banks 2, 3
+        }
+        case 16: return uint32_t(__builtin_amdgcn_ds_swizzle(int(v), 0x401F));  // and 0x1F, xor 0x10
+        default: return uint32_t(__shfl_xor(int(v), m));
+    }
+}
 __device__ __forceinline__ uint64_t shfl_xor_v(uint64_t v, int m) {
-    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), m));
-    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), m));
+    const uint32_t lo = lane_xor(uint32_t(v), m);
+    const uint32_t hi = lane_xor(uint32_t(v >> 32), m);
     return (uint64_t(hi) << 32) | lo;
 }
-__device__ __forceinline__ uint32_t shfl_xor_v(uint32_t v, int m) { return uint32_t(__shfl_xor(int(v), m)); }
+__device__ __forceinline__ uint32_t shfl_xor_v(uint32_t v, int m) { return lane_xor(v, m); }
 
 static_assert(kSmjCap == 4 * kJoinThreads, "SMJ sort tiles hold up to 4 elements per thread");
 
