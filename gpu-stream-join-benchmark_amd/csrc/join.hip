@@ -697,19 +697,20 @@ __device__ __forceinline__ void agg_add(AggLds &t, uint32_t k, uint32_t n, uint3
     if (rest) atomicAdd(reinterpret_cast<unsigned long long *>(sum), (unsigned long long)rest);
 }
 
-// Inserts one record per lane (`valid` lanes).  When several lanes of the
-// wave share the first valid lane's key (the common case under heavy skew,
-// where one key dominates a partition), they are combined into one update.
+// Inserts one record per lane (`valid` lanes); called by every lane of the
+// wave.  When several lanes of the wave share the first valid lane's key
+// (the common case under heavy skew, where one key dominates a partition),
+// they are combined into one update.
 __device__ __forceinline__ void agg_insert(AggLds &t, bool valid, Rec r, int bshift) {
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int lead = __ffsll((unsigned long long)vmask) - 1;
-    const uint32_t k0 = __shfl(r.x, lead, kWave);
+    const uint32_t k0 = uint32_t(__builtin_amdgcn_readlane(int(r.x), lead));
     const bool same = valid && r.x == k0;
     const uint64_t smask = __ballot(same);
     if (__popcll(smask) > 1) {  // wave-uniform
-        const uint64_t sa = wave_sum(same && lane != lead ? hash_a(r.x, r.y) : 0ull);
+        const uint64_t sa = wave_sum_full(same && lane != lead ? hash_a(r.x, r.y) : 0ull);
         if (lane == lead) agg_add(t, k0, uint32_t(__popcll(smask)), r.y, sa, bshift);
         if (valid && !same) agg_add(t, r.x, 1u, r.y, 0ull, bshift);
     } else if (valid) {
