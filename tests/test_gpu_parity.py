@@ -93,6 +93,42 @@ def test_materialised_pairs(ctx, torch_mod, orc, alg, method):
     assert np.array_equal(sort_rows(rows), ref_rows)
 
 
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+@pytest.mark.parametrize("size", [((1 << 19) - 5, (1 << 19) + 5), ((1 << 19) + 3, 1 << 19)])
+def test_small_window_boundary(ctx, torch_mod, orc, alg, size):
+    """Windows of exactly 2^20 tuples (both sides) take 2048-tuple partition
+    chunks and the small-window fanout; three more tuples take the 16384-
+    tuple chunks of large windows.  Both give the oracle's triple."""
+    nL, nR = size
+    print(f"case boundary alg={alg} size={size}", flush=True)
+    R, S = make(nL, 1 << 22, 500 + nL), make(nR, 1 << 22, 600 + nR)
+    o, _ = orc.lazy_hj(R, S, 1 << 16)
+    st = run_count(ctx, torch_mod, alg, R, S)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    st = run_count(ctx, torch_mod, alg, R, S, F_GROUPED)
+    assert (st.result_count, st.checksum1) == (o.count, o.c1)
+
+
+@pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
+def test_small_context_windows(sjbmod, torch_mod, orc, alg):
+    """A context made for 2^16-tuple windows (its offsets buffer sized for
+    the 2048-tuple chunks of small windows): full, ragged and tiny windows,
+    plain and asynchronous, give the oracle's triple."""
+    W = 1 << 16
+    with sjbmod.Context(device=0, max_window=W, distinct_keys=1 << 16) as c:
+        for nL, nR in [(W, W), (W - 1, 1000), (3, W)]:
+            R, S = make(nL, 1 << 16, 700 + nL), make(nR, 1 << 16, 800 + nR)
+            o, _ = orc.lazy_hj(R, S, 1 << 16)
+            st = run_count(c, torch_mod, alg, R, S)
+            assert (st.result_count, st.checksum1, st.checksum2) == o.triple(), (nL, nR)
+            dR, dS = to_dev(torch_mod, R), to_dev(torch_mod, S)
+            d = torch_mod.zeros(32, dtype=torch_mod.uint8, device="cuda")
+            c.lazy_join_async(alg, F_C2, dR.data_ptr(), nL, dS.data_ptr(), nR, d.data_ptr())
+            c.sync()
+            r = d.cpu().numpy().view(sjbmod.WRES_DTYPE)[0]
+            assert (int(r["count"]), int(r["checksum1"]), int(r["checksum2"])) == o.triple(), (nL, nR)
+
+
 @pytest.mark.parametrize("key_range", [16, 1000, 1 << 24])
 @pytest.mark.parametrize("size", [(1, 1), (100, 37), (1023, 1025), (5000, 4096), (16389, 20011)])
 def test_bitmap_pairs(ctx, torch_mod, orc, key_range, size):
