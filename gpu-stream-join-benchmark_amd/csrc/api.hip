@@ -118,7 +118,7 @@ static bool valid_alg(int alg) { return alg == SJB_ALG_NLJ || alg == SJB_ALG_HJ 
 static bool valid_method(int m) { return m >= SJB_OUT_NOOUTPUT && m <= SJB_OUT_ATOMIC; }
 
 // Plans one HJ/SMJ window (fanout, partition function) and enqueues its
-// partition pass (plus the range parameters or the heavy-key reset it
+// partition pass (plus the key-range sample or the heavy-key reset it
 // needs).  Both sides non-empty.
 int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, uint64_t nL,
                      const sjb_tuple_t *R, uint64_t nR, int write, WindowPlan *out, int buf,
@@ -139,6 +139,7 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.payload = write ? PAY_IDX : PAY_VAL;
     p.peel = learned && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
     p.chunk = small ? kSmallChunk : kChunk;
+    p.range_blocks = 0;
     p.ncL = int((nL + p.chunk - 1) / p.chunk);
     p.ncR = int((nR + p.chunk - 1) / p.chunk);
     if (p.ncL > kMaxChunks || p.ncR > kMaxChunks) return SJB_E_CAPACITY;
@@ -146,7 +147,7 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.direct = direct && !write && !p.peel && p.kind != PART_LEARNED ? direct : nullptr;
     int rc;
     if (p.kind == PART_RANGE) {
-        rc = launch_minmax(c, p.ps, L, nL, R, nR, p.logF);
+        rc = launch_minmax(c, p.ps, L, nL, R, nR, &p.range_blocks);
         if (rc) return rc;
     }
     if (p.peel) {
@@ -421,7 +422,6 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
         (rc = dev_alloc(c, (void **)&c->d_scratch, 4096 * sizeof(uint32_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_partials, 3 * n_partials * sizeof(uint64_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_result, 64 * sizeof(sjb_window_result_t))) ||
-        (rc = dev_alloc(c, (void **)&c->d_pparams, sizeof(PartParams))) ||
         (rc = dev_alloc(c, (void **)&c->d_rmi, sizeof(RmiParams))) ||
         (rc = dev_alloc(c, (void **)&c->d_heavy, 4 * kMaxHeavy * sizeof(uint64_t))) ||
         (rc = dev_alloc(c, (void **)&c->d_cursor, 4 * sizeof(uint64_t))) ||

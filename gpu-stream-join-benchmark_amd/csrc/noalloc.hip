@@ -3,7 +3,7 @@
 //
 // It runs the same partition, join and reduction kernels as
 // sjb_lazy_join_async, but their scratch memory (partition records, chunk
-// offsets, per-partition partials, range-partition parameters) is carved out
+// offsets, per-partition partials, the key sample's range bounds) is carved out
 // of one file-scope device array, `scratch`, instead of a context's
 // hipMalloc'd buffers.  Each GPU holds its own copy of the array for the
 // life of the process; its device address is looked up once per device with
@@ -32,8 +32,7 @@ constexpr uint64_t kOffPart = 0;
 constexpr uint64_t kOffOffs = align256(kOffPart + 2 * kNaMax * sizeof(Rec));
 constexpr uint64_t kOffPartials = align256(kOffOffs + kNaOffsWords * sizeof(uint32_t));
 constexpr uint64_t kOffMinmax = align256(kOffPartials + 3 * kNaPartials * sizeof(uint64_t));
-constexpr uint64_t kOffPparams = align256(kOffMinmax + kNaMinmaxWords * sizeof(uint32_t));
-constexpr uint64_t kNaBytes = align256(kOffPparams + sizeof(PartParams));
+constexpr uint64_t kNaBytes = align256(kOffMinmax + kNaMinmaxWords * sizeof(uint32_t));
 
 }  // namespace
 
@@ -111,7 +110,6 @@ extern "C" int sjb_lazy_join_noalloc(int algorithm, uint32_t flags, const sjb_tu
     c.d_offs = reinterpret_cast<uint32_t *>(d->ptr + kOffOffs);
     c.d_partials = reinterpret_cast<uint64_t *>(d->ptr + kOffPartials);
     c.d_scratch = reinterpret_cast<uint32_t *>(d->ptr + kOffMinmax);
-    c.d_pparams = reinterpret_cast<PartParams *>(d->ptr + kOffPparams);
     int logF = 0;
     return window_enqueue(&c, algorithm, flags, d_left, n_left, d_right, n_right, 0, nullptr, 0, 0, d_result,
                           &logF, nullptr);

@@ -177,7 +177,7 @@ struct PartFn {
 // equal keys still meet; the sample only sets the balance.  The sample is
 // spread over many workgroups (one key per thread: the loads of one
 // workgroup alone would queue behind each other), each writing its partial
-// bounds; k_minmax_final folds them.
+// bounds; the range partition pass folds them (range_params).
 constexpr int kMinmaxThreads = 1024;
 constexpr uint32_t kMinmaxSample = 1u << 16;
 __global__ __launch_bounds__(kMinmaxThreads) void k_minmax_partial(const sjb_tuple_t *__restrict__ L, uint64_t nL,
@@ -209,37 +209,37 @@ __global__ __launch_bounds__(kMinmaxThreads) void k_minmax_partial(const sjb_tup
     partial[2 * blockIdx.x + 1] = mx;
 }
 
-// One workgroup folds the per-block partials into kmin / shift.
-__global__ __launch_bounds__(kMinmaxThreads) void k_minmax_final(const uint32_t *__restrict__ partial,
-                                                                 int nblocks, int logF,
-                                                                 PartParams *__restrict__ pp) {
-    __shared__ uint32_t smin[kMinmaxThreads / kWave], smax[kMinmaxThreads / kWave];
-    uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kMinmaxThreads) {
-        mn = min(mn, partial[2 * b]);
-        mx = max(mx, partial[2 * b + 1]);
-    }
-    wave_minmax_full(mn, mx);
-    const int wid = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        smin[wid] = mn;
-        smax[wid] = mx;
+// Range-partition parameters (kmin, shift) from the nblocks (min, max)
+// pairs of k_minmax_partial: every partition workgroup folds them itself —
+// its first wave, DPP min/max — so no reduction kernel runs between the
+// sample and the partition pass.  Every thread of the block (one barrier).
+__device__ __forceinline__ void range_params(const uint32_t *__restrict__ partial, int nblocks, int logF,
+                                             uint32_t *kmin, uint32_t *shift) {
+    __shared__ uint32_t s_rp[2];
+    if (threadIdx.x < kWave) {  // the whole first wave
+        uint32_t mn = 0xFFFFFFFFu, mx = 0;
+        for (int b = threadIdx.x; b < nblocks; b += kWave) {
+            mn = min(mn, partial[2 * b]);
+            mx = max(mx, partial[2 * b + 1]);
+        }
+        wave_minmax_full(mn, mx);
+        if (threadIdx.x == 0) {
+            if (mn > mx) mn = mx = 0;
+            const uint32_t span = mx - mn;
+            const int bits = span ? 32 - __clz(span) : 0;
+            s_rp[0] = mn;
+            s_rp[1] = bits > logF ? uint32_t(bits - logF) : 0u;
+        }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < kMinmaxThreads / kWave; w++) {
-        mn = min(mn, smin[w]);
-        mx = max(mx, smax[w]);
-    }
-    if (mn > mx) mn = mx = 0;
-    uint32_t span = mx - mn;
-    int bits = span ? 32 - __clz(span) : 0;
-    pp->kmin = mn;
-    pp->shift = bits > logF ? uint32_t(bits - logF) : 0u;
+    *kmin = s_rp[0];
+    *shift = s_rp[1];
 }
 
+// Samples the window's keys (k_minmax_partial) into c->d_scratch; returns
+// the number of (min, max) pairs the range partition pass folds.
 int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
-                  uint64_t nR, int logF) {
+                  uint64_t nR, int *blocks_out) {
     const uint64_t n = nL + nR;
     const uint32_t ns = uint32_t(std::min<uint64_t>(n, kMinmaxSample));
     const int blocks = std::max(1, int((ns + kMinmaxThreads - 1) / kMinmaxThreads));
@@ -248,8 +248,7 @@ int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, 
     // The scan-block buffer doubles as scratch for the partial bounds.
     k_minmax_partial<<<blocks, kMinmaxThreads, 0, s>>>(L, nL, R, ns, step, frac, c->d_scratch);
     SJB_HIP_TRY(hipGetLastError());
-    k_minmax_final<<<1, kMinmaxThreads, 0, s>>>(c->d_scratch, blocks, logF, c->d_pparams);
-    SJB_HIP_TRY(hipGetLastError());
+    *blocks_out = blocks;
     return SJB_OK;
 }
 
@@ -399,15 +398,17 @@ __device__ __forceinline__ void partition_chunk(const sjb_tuple_t *__restrict__ 
 template <int KIND, int PAY, bool PEEL, int CH>
 __global__ __launch_bounds__(kPartThreads) void k_partition(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, uint64_t *__restrict__ heavy_acc,
-    sjb_window_result_t *__restrict__ direct) {
+    uint64_t nR, int logF, const uint32_t *__restrict__ range_partial, int range_blocks,
+    const RmiParams *__restrict__ rmi, uint32_t *__restrict__ offs, Rec *__restrict__ part_out,
+    uint64_t *__restrict__ heavy_acc, sjb_window_result_t *__restrict__ direct) {
     extern __shared__ uint32_t lds[];
     __shared__ HeavySet hs[1];
     if (direct && blockIdx.x == 0 && threadIdx.x < sizeof(sjb_window_result_t) / 8)
         reinterpret_cast<uint64_t *>(direct)[threadIdx.x] = 0;
+    uint32_t kmin = 0, shift = 0;
+    if constexpr (KIND == PART_RANGE) range_params(range_partial, range_blocks, logF, &kmin, &shift);
     PartFn<KIND> part;
-    part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
+    part.init(logF, kmin, shift, rmi);
     const bool left = int(blockIdx.x) < ncL;
     const int chunk = left ? blockIdx.x : blockIdx.x - ncL;
     const uint64_t n = left ? nL : nR;
@@ -565,8 +566,9 @@ __device__ __forceinline__ void chunk_step(const PartFn<KIND> &part, int F, uint
 template <int KIND, int PAY>
 __global__ __launch_bounds__(kPartThreads) void k_partition_pipe(
     const sjb_tuple_t *__restrict__ L, uint64_t nL, int ncL, const sjb_tuple_t *__restrict__ R,
-    uint64_t nR, int ncR, int logF, const PartParams *__restrict__ pp, const RmiParams *__restrict__ rmi,
-    uint32_t *__restrict__ offs, Rec *__restrict__ part_out, sjb_window_result_t *__restrict__ direct) {
+    uint64_t nR, int ncR, int logF, const uint32_t *__restrict__ range_partial, int range_blocks,
+    const RmiParams *__restrict__ rmi, uint32_t *__restrict__ offs, Rec *__restrict__ part_out,
+    sjb_window_result_t *__restrict__ direct) {
     extern __shared__ uint32_t lds[];
     const int F = 1 << logF;
     // the join kernel accumulates into `direct`; it starts from zero here
@@ -575,15 +577,18 @@ __global__ __launch_bounds__(kPartThreads) void k_partition_pipe(
     uint32_t *hist = lds;       // F + 1 counters
     uint32_t *ws = lds + F + 2; // wave sums
     Rec *stage = reinterpret_cast<Rec *>(lds + ((F + 2 + kPartThreads / kWave + 1) & ~1));
-    PartFn<KIND> part;
-    part.init(logF, KIND == PART_RANGE ? pp->kmin : 0u, KIND == PART_RANGE ? pp->shift : 0u, rmi);
     const int nch = ncL + ncR;
     const int G = int(gridDim.x);
     int b = blockIdx.x;
-    if (b >= nch) return;
+    if (b >= nch) return;  // the whole workgroup
     ChunkGeo ga = chunk_geo(b, L, nL, ncL, R, nR);
     Rec ra[kPartPer], rb[kPartPer];
     chunk_load<PAY>(ga, ra);
+    // range parameters folded while the first chunk's loads are in flight
+    uint32_t kmin = 0, shift = 0;
+    if constexpr (KIND == PART_RANGE) range_params(range_partial, range_blocks, logF, &kmin, &shift);
+    PartFn<KIND> part;
+    part.init(logF, kmin, shift, rmi);
     for (;;) {
         const int b1 = b + G;
         const bool h1 = b1 < nch;
@@ -751,10 +756,12 @@ static int partition_chunks(const WindowPlan &p, const sjb_tuple_t *L, const sjb
     const int nblk = p.ncL + p.ncR;
     if (p.payload == PAY_VAL)
         k_partition<KIND, PAY_VAL, PEEL, CH><<<nblk, kPartThreads, lds, p.ps>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy, p.direct);
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_scratch, p.range_blocks, c->d_rmi, p.offs, p.part, p.heavy,
+            p.direct);
     else
         k_partition<KIND, PAY_IDX, false, CH><<<nblk, kPartThreads, lds, p.ps>>>(
-            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.heavy, p.direct);
+            L, p.nL, p.ncL, R, p.nR, p.logF, c->d_scratch, p.range_blocks, c->d_rmi, p.offs, p.part, p.heavy,
+            p.direct);
     SJB_HIP_TRY(hipGetLastError());
     return SJB_OK;
 }
@@ -771,7 +778,8 @@ static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
         if (p.payload == PAY_VAL) {
             const int grid = std::min(nblk, std::max(c->n_cu, 1));
             k_partition_pipe<KIND, PAY_VAL><<<grid, kPartThreads, lds, p.ps>>>(
-                L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_pparams, c->d_rmi, p.offs, p.part, p.direct);
+                L, p.nL, p.ncL, R, p.nR, p.ncR, p.logF, c->d_scratch, p.range_blocks, c->d_rmi, p.offs, p.part,
+                p.direct);
             SJB_HIP_TRY(hipGetLastError());
             return SJB_OK;
         }

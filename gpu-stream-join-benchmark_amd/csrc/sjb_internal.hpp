@@ -109,12 +109,6 @@ struct RmiParams {
     uint32_t hfilter[128];
 };
 
-// Device parameters of range partitioning (written by k_minmax_final).
-struct PartParams {
-    uint32_t kmin;        // lower key bound (smallest sampled key of both sides)
-    uint32_t shift;       // key bits dropped
-};
-
 // One window of a batched launch (device descriptor, built on the host).
 // Records of the window's left side are partitioned into
 // d_part[lpart, lpart + nL), right side into d_part[rpart, rpart + nR).
@@ -167,6 +161,7 @@ struct WindowPlan {
     uint32_t flags;
     uint64_t nL, nR;
     int chunk;     // tuples per partition chunk: kChunk or kSmallChunk
+    int range_blocks;  // range kind: (min, max) pairs of the key sample in ctx d_scratch
     int ncL, ncR;  // chunks per side
     bool peel;     // learned + grouped: heavy keys aggregated apart
     // where the partition pass runs and writes (the ctx's own buffers and
@@ -206,7 +201,6 @@ struct sjb_ctx {
     uint64_t *d_partials = nullptr;   // 4 u64 per partition
     sjb_window_result_t *d_result = nullptr;
     sjb_window_result_t *h_result = nullptr;  // pinned
-    sjb::PartParams *d_pparams = nullptr;
     sjb::RmiParams *d_rmi = nullptr;
     bool rmi_valid = false;
     uint32_t rmi_nheavy = 0;          // host copy of d_rmi->n_heavy
@@ -289,7 +283,7 @@ int dev_regrow(sjb_ctx *c, void **p, size_t old_bytes, size_t bytes);
 // ---- launchers (partition.hip) ----
 int choose_logF(int alg, uint32_t flags, uint64_t n_build, int requested, bool small);
 int launch_minmax(sjb_ctx *c, hipStream_t s, const sjb_tuple_t *L, uint64_t nL, const sjb_tuple_t *R,
-                  uint64_t nR, int logF);
+                  uint64_t nR, int *blocks_out);
 int launch_partition(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L, const sjb_tuple_t *R);
 int partition_setup(sjb_ctx *c);
 int launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, uint64_t n);
