@@ -138,7 +138,7 @@ int window_partition(sjb_ctx *c, int alg, uint32_t flags, const sjb_tuple_t *L, 
     p.kind = part_kind(alg, learned);
     p.payload = write ? PAY_IDX : PAY_VAL;
     p.peel = learned && (flags & SJB_F_GROUPED) && !write && c->rmi_nheavy > 0;
-    p.chunk = small ? kSmallChunk : kChunk;
+    p.chunk = !small ? kChunk : nL + nR <= kTinyWindow ? kTinyChunk : kSmallChunk;
     p.range_blocks = 0;
     p.ncL = int((nL + p.chunk - 1) / p.chunk);
     p.ncR = int((nR + p.chunk - 1) / p.chunk);
@@ -411,7 +411,9 @@ int sjb_open(int device, const sjb_options_t *opts, sjb_ctx_t **out) {
     const uint64_t nchunks = std::min<uint64_t>((cap + kChunk - 1) / kChunk, kMaxChunks);
     // offsets rows: two sides of kChunk chunks, or a small window's
     // kSmallChunk chunks (both sides together at most kSmallWindow tuples)
-    const uint64_t small_rows = std::min<uint64_t>(2 * cap, kSmallWindow) / kSmallChunk + 2;
+    // or kTinyChunk chunks (at most kTinyWindow tuples)
+    const uint64_t small_rows = std::max<uint64_t>(std::min<uint64_t>(2 * cap, kSmallWindow) / kSmallChunk,
+                                                   std::min<uint64_t>(2 * cap, kTinyWindow) / kTinyChunk) + 2;
     c->offs_cap = ((uint64_t(1) << kMaxLogF) + 1) * std::max<uint64_t>(2 * std::max<uint64_t>(nchunks, 1), small_rows);
     const uint64_t n_partials = std::max<uint64_t>(65536, 1ull << kMaxLogF);
     c->mask_cap = std::max<uint64_t>(cap, 1ull << 20);

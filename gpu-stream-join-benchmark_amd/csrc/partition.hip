@@ -727,7 +727,7 @@ int batch_log_fanout(int alg, uint32_t flags, uint64_t n_build, int bias) {
 // Fanout of a lone window: the tile rule above, raised to a floor while
 // partitions keep >= 64 build records — a small window, or one with few
 // distinct keys and many matches, would otherwise run its join on a few
-// dozen workgroups.  Windows in kSmallChunk chunks (small): ~1024 build
+// dozen workgroups.  Windows in kSmallChunk or kTinyChunk chunks (small): ~1024 build
 // records per partition and at least 2^7 partitions, at most 2^9 (measured
 // best at 2^14..2^19 tuples per side: 2^7 up to 2^16, 2^8 at 2^18, 2^9 at
 // 2^19); windows in kChunk chunks: 2^9 partitions (two join workgroups per
@@ -770,6 +770,7 @@ template <int KIND, bool PEEL>
 static int partition_kind(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L,
                           const sjb_tuple_t *R) {
     if (p.chunk == kSmallChunk) return partition_chunks<KIND, PEEL, kSmallChunk>(p, L, R, c);
+    if (p.chunk == kTinyChunk) return partition_chunks<KIND, PEEL, kTinyChunk>(p, L, R, c);
     if (p.chunk != kChunk) return SJB_E_INVALID;
     const size_t lds = partition_lds_bytes(p.logF);
     const int nblk = p.ncL + p.ncR;

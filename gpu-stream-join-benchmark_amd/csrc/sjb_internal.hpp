@@ -52,12 +52,22 @@ typedef uint2 Rec;
 #ifndef SJB_SMALL_WINDOW
 #define SJB_SMALL_WINDOW (1 << 20)
 #endif
+#ifndef SJB_TINY_CHUNK
+#define SJB_TINY_CHUNK 1024
+#endif
+#ifndef SJB_TINY_WINDOW
+#define SJB_TINY_WINDOW (1 << 17)
+#endif
 constexpr int kChunk = SJB_CHUNK;              // tuples per partition-pass workgroup
 // Windows of at most kSmallWindow tuples (both sides) are partitioned in
-// kSmallChunk-tuple chunks, one workgroup each: in kChunk chunks a 2^17-
-// tuple window would keep 8 of the chip's workgroups busy.
+// kSmallChunk-tuple chunks, one workgroup each, and those of at most
+// kTinyWindow in kTinyChunk chunks: in kChunk chunks a 2^17-tuple window
+// would keep 8 of the chip's workgroups busy.  (Smaller chunks mean longer
+// partition views in the join, so larger windows keep larger chunks.)
 constexpr int kSmallChunk = SJB_SMALL_CHUNK;
 constexpr uint64_t kSmallWindow = SJB_SMALL_WINDOW;
+constexpr int kTinyChunk = SJB_TINY_CHUNK;
+constexpr uint64_t kTinyWindow = SJB_TINY_WINDOW;
 constexpr int kPartThreads = SJB_PART_THREADS; // threads of the partition pass
 constexpr int kMaxLogF = 12;                   // fanout bound: 4096 partitions
 constexpr int kMaxChunks = 1024;               // chunks per side
@@ -160,7 +170,7 @@ struct WindowPlan {
     int payload;   // Payload
     uint32_t flags;
     uint64_t nL, nR;
-    int chunk;     // tuples per partition chunk: kChunk or kSmallChunk
+    int chunk;     // tuples per partition chunk: kChunk, kSmallChunk or kTinyChunk
     int range_blocks;  // range kind: (min, max) pairs of the key sample in ctx d_scratch
     int ncL, ncR;  // chunks per side
     bool peel;     // learned + grouped: heavy keys aggregated apart

@@ -94,11 +94,13 @@ def test_materialised_pairs(ctx, torch_mod, orc, alg, method):
 
 
 @pytest.mark.parametrize("alg", [HJ, SMJ])
-@pytest.mark.parametrize("size", [((1 << 19) - 5, (1 << 19) + 5), ((1 << 19) + 3, 1 << 19)])
+@pytest.mark.parametrize("size", [((1 << 19) - 5, (1 << 19) + 5), ((1 << 19) + 3, 1 << 19),
+                                  ((1 << 16) - 1, (1 << 16) + 1), ((1 << 16) + 1, 1 << 16)])
 def test_small_window_boundary(ctx, torch_mod, orc, alg, size):
-    """Windows of exactly 2^20 tuples (both sides) take 2048-tuple partition
-    chunks and the small-window fanout; three more tuples take the 16384-
-    tuple chunks of large windows.  Both give the oracle's triple."""
+    """Partition chunk sizes switch with the window's total size: exactly
+    2^20 tuples (both sides) take 2048-tuple chunks and three more the
+    16384-tuple chunks of large windows; exactly 2^17 take 1024-tuple chunks
+    and one more 2048-tuple chunks.  All give the oracle's triple."""
     nL, nR = size
     print(f"case boundary alg={alg} size={size}", flush=True)
     R, S = make(nL, 1 << 22, 500 + nL), make(nR, 1 << 22, 600 + nR)
@@ -112,8 +114,8 @@ def test_small_window_boundary(ctx, torch_mod, orc, alg, size):
 @pytest.mark.parametrize("alg", [HJ, SMJ, NLJ])
 def test_small_context_windows(sjbmod, torch_mod, orc, alg):
     """A context made for 2^16-tuple windows (its offsets buffer sized for
-    the 2048-tuple chunks of small windows): full, ragged and tiny windows,
-    plain and asynchronous, give the oracle's triple."""
+    the 1024- and 2048-tuple chunks of small windows): full, ragged and tiny
+    windows, plain and asynchronous, give the oracle's triple."""
     W = 1 << 16
     with sjbmod.Context(device=0, max_window=W, distinct_keys=1 << 16) as c:
         for nL, nR in [(W, W), (W - 1, 1000), (3, W)]:
