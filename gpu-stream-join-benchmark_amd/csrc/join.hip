@@ -87,6 +87,10 @@ struct PartView {
         }
         return run_index(pref[side], base[side], nc[side], j);
     }
+    // index() for j known to be below idcap.
+    __device__ __forceinline__ uint32_t index_tab(int side, uint32_t j) const {
+        return base[side][runid[side][j]] + j;
+    }
 };
 
 // Dynamic LDS bytes of a PartView: base/pref for both sides plus two
@@ -552,19 +556,22 @@ __device__ __forceinline__ void hj_join_part(const JoinIO &io, const PartView &v
     HjTable &t = m.t;
     const Rec *__restrict__ part = io.part;
     for (int i = threadIdx.x; i <= kHjNB; i += kJoinThreads) t.off[i] = 0;
+    // The first tile and probe pass lie inside the run-id table: an HJ view
+    // holds kHjCap entries per side (idcap_of, join_idcap).
+    static_assert(kHjProbe <= kHjCap, "first probe pass must fit the run-id table");
     const uint32_t tn0 = min<uint32_t>(nb, kHjCap), pn0 = min<uint32_t>(np, kHjProbe);
     Rec rb[kHjPerT];
     Rec rp[kHjPerP];
 #pragma unroll
     for (int k = 0; k < kHjPerT; k++) {
         const uint32_t i = k * kJoinThreads + threadIdx.x;
-        if (i < tn0) rb[k] = part[v.index(0, i)];
+        if (i < tn0) rb[k] = part[v.index_tab(0, i)];
     }
 #pragma unroll
     for (int k = 0; k < kHjPerP; k++) {
         const uint32_t j = k * kJoinThreads + threadIdx.x;
         rp[k] = make_uint2(0, 0);
-        if (j < pn0) rp[k] = part[v.index(1, j)];
+        if (j < pn0) rp[k] = part[v.index_tab(1, j)];
     }
     if constexpr (WRITE) {
 #pragma unroll
