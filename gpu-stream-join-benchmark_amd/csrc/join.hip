@@ -170,13 +170,24 @@ __device__ __forceinline__ PartView load_view(uint32_t *dyn, const uint32_t *__r
     return v;
 }
 
-// load_view when ncL + ncR <= blockDim and ncL is a whole number of waves
-// (every window of up to 2^22 tuples per side with 64-aligned chunk counts,
-// e.g. 2^22 = 256 chunks): one slot per thread, one scan, the left total
-// from the wave sums, three barriers instead of six.  Ends with a barrier.
-__device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_t *__restrict__ offs, int logF,
-                                                    int p, int ncL, int ncR, uint32_t lbase, uint32_t rbase,
-                                                    uint32_t chunk, uint32_t idcap, uint32_t *ws) {
+// Thread q's run bounds (a, b) of partition p for load_view_small: slot q <
+// ncL is left chunk q, slot ncL + c right chunk c; zero past the slots.
+__device__ __forceinline__ void view_bounds(const uint32_t *__restrict__ offs, int logF, int p, int nq,
+                                            uint32_t &a, uint32_t &b) {
+    a = b = 0;
+    const int q = threadIdx.x;
+    if (q < nq) {
+        const uint32_t *o = offs + uint64_t(q) * ((1 << logF) + 1);
+        a = o[p];
+        b = o[p + 1];
+    }
+}
+
+// The view of load_view_small from bounds loaded by view_bounds.  Ends with
+// a barrier.
+__device__ __forceinline__ PartView view_small_from(uint32_t *dyn, uint32_t a, uint32_t b, int ncL, int ncR,
+                                                    uint32_t lbase, uint32_t rbase, uint32_t chunk,
+                                                    uint32_t idcap, uint32_t *ws) {
     PartView v;
     v.nc[0] = ncL;
     v.nc[1] = ncR;
@@ -191,12 +202,6 @@ __device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_
     const int q = threadIdx.x;
     const bool right = q >= ncL;
     const int c = right ? q - ncL : q;
-    uint32_t a = 0, b = 0;
-    if (q < nq) {
-        const uint32_t *o = offs + uint64_t(q) * ((1 << logF) + 1);
-        a = o[p];
-        b = o[p + 1];
-    }
     uint32_t wt;
     const uint32_t ex = wave_excl_scan_full(b - a, &wt);
     const int wid = threadIdx.x / kWave;
@@ -229,6 +234,18 @@ __device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_
     }
     __syncthreads();
     return v;
+}
+
+// load_view when ncL + ncR <= blockDim and ncL is a whole number of waves
+// (every window of up to 2^22 tuples per side with 64-aligned chunk counts,
+// e.g. 2^22 = 256 chunks): one slot per thread, one scan, the left total
+// from the wave sums, three barriers instead of six.  Ends with a barrier.
+__device__ __forceinline__ PartView load_view_small(uint32_t *dyn, const uint32_t *__restrict__ offs, int logF,
+                                                    int p, int ncL, int ncR, uint32_t lbase, uint32_t rbase,
+                                                    uint32_t chunk, uint32_t idcap, uint32_t *ws) {
+    uint32_t a, b;
+    view_bounds(offs, logF, p, ncL + ncR, a, b);
+    return view_small_from(dyn, a, b, ncL, ncR, lbase, rbase, chunk, idcap, ws);
 }
 
 __device__ __forceinline__ void write_row(sjb_result_tuple_t *__restrict__ out, uint64_t pos,
@@ -1398,6 +1415,33 @@ __host__ __device__ constexpr int idcap_of() {
 // ---------------------------------------------------------------------------
 // one window: one workgroup per partition
 // ---------------------------------------------------------------------------
+// Block sum of a workgroup's (count, C1, C2) with a single barrier (nothing
+// runs after it): to partials[3p ...], or added to `direct` when it is set.
+__device__ __forceinline__ void join_tail(Acc acc, uint64_t *s_red, uint64_t *__restrict__ partials, int p,
+                                          sjb_window_result_t *__restrict__ direct) {
+    acc.count = wave_sum_full(acc.count);
+    acc.c1 = wave_sum_full(acc.c1);
+    acc.c2 = wave_sum_full(acc.c2);
+    constexpr int kW = kJoinThreads / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int w = threadIdx.x / kWave;
+        s_red[w] = acc.count;
+        s_red[kW + w] = acc.c1;
+        s_red[2 * kW + w] = acc.c2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kW; w++) t += s_red[threadIdx.x * kW + w];
+        // sums mod 2^64: the atomics give the same bits in any order
+        if (!direct)
+            partials[3 * p + threadIdx.x] = t;
+        else if (t)
+            atomicAdd(reinterpret_cast<unsigned long long *>(direct) + threadIdx.x, (unsigned long long)t);
+    }
+}
+
 // The workgroup's (count, C1, C2) goes to partials[3p ...] and is summed
 // by k_reduce, or, when `direct` is set (the partition pass zeroed it), is
 // added to the window result with three 64-bit atomics.
@@ -1424,28 +1468,40 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
     Acc acc;
     acc.zero();
     join_part<ALG, C2, WRITE, GROUPED, true>(io, v, bshift, s_mem, s_ws, &s_base, acc);
-    // block sum with a single barrier (nothing runs after it)
-    acc.count = wave_sum_full(acc.count);
-    acc.c1 = wave_sum_full(acc.c1);
-    acc.c2 = wave_sum_full(acc.c2);
-    constexpr int kW = kJoinThreads / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const int w = threadIdx.x / kWave;
-        s_red[w] = acc.count;
-        s_red[kW + w] = acc.c1;
-        s_red[2 * kW + w] = acc.c2;
+    join_tail(acc, s_red, partials, p, direct);
+}
+
+// Count-path HJ join of one window on a resident grid (the host launches it
+// when the partition pass zeroed `direct` and load_view_small applies):
+// workgroup b joins partitions xcd_partition(b), xcd_partition(b + G), ...
+// (G a multiple of the 8 XCDs, so all of them on b's XCD), and loads the
+// next partition's run bounds while the current one's records are gathered,
+// so that global round trip leaves the per-partition chain.  Its
+// (count, C1, C2) over all of them is added to the window result once.
+template <bool C2>
+__global__ __launch_bounds__(kJoinThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_join_pipe(
+    JoinIO io, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR, uint32_t nL, int chunk, int bshift,
+    sjb_window_result_t *__restrict__ direct) {
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<SJB_ALG_HJ, false, false>::value];
+    __shared__ uint32_t s_ws[kJoinThreads / kWave];
+    __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
+    __shared__ unsigned long long s_base;
+    extern __shared__ uint32_t dyn[];
+
+    const int F = 1 << logF, G = int(gridDim.x), nq = ncL + ncR;
+    int it = blockIdx.x;
+    uint32_t a, b;
+    view_bounds(offs, logF, xcd_partition(it, F), nq, a, b);
+    Acc acc;
+    acc.zero();
+    for (; it < F; it += G) {
+        __syncthreads();  // the previous partition's LDS is no longer read
+        const PartView v = view_small_from(dyn, a, b, ncL, ncR, 0u, nL, uint32_t(chunk), kHjCap, s_ws);
+        if (it + G < F) view_bounds(offs, logF, xcd_partition(it + G, F), nq, a, b);
+        hj_join_part<C2, false, false>(io, v, bshift, *reinterpret_cast<HjLds<false> *>(s_mem), s_ws, &s_base,
+                                       acc);
     }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        uint64_t t = 0;
-#pragma unroll
-        for (int w = 0; w < kW; w++) t += s_red[threadIdx.x * kW + w];
-        // sums mod 2^64: the atomics give the same bits in any order
-        if (!direct)
-            partials[3 * p + threadIdx.x] = t;
-        else if (t)
-            atomicAdd(reinterpret_cast<unsigned long long *>(direct) + threadIdx.x, (unsigned long long)t);
-    }
+    join_tail(acc, s_red, nullptr, 0, direct);
 }
 
 // ---------------------------------------------------------------------------
@@ -1597,7 +1653,21 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     io.out_cap = out_cap;
     io.wrap = wrap;
     io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
-#define SJB_J(C2_, W_, G_)                                                                          \
+    if (ALG == SJB_ALG_HJ && p.payload == PAY_VAL && !grouped && p.direct && p.ncL + p.ncR <= kJoinThreads &&
+        p.ncL % kWave == 0) {
+        // resident grid: three workgroups per CU, a multiple of the 8 XCDs
+        const int slots = c->n_cu > 0 ? 3 * c->n_cu : F;
+        const int G = F <= slots ? F : std::max(slots & ~7, 8);
+        if (c2)
+            k_join_pipe<true><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
+                                                                    uint32_t(p.nL), p.chunk, bshift, p.direct);
+        else
+            k_join_pipe<false><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
+                                                                     uint32_t(p.nL), p.chunk, bshift, p.direct);
+        SJB_HIP_TRY(hipGetLastError());
+        return SJB_OK;
+    }
+#define SJB_J(C2_, W_, G_)                                                                        \
     k_join<ALG, C2_, W_, G_><<<F, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL,      \
                                                                  p.ncR, uint32_t(p.nL), p.chunk, bshift, \
                                                                  c->d_partials, p.direct)
