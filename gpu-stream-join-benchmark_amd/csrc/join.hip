@@ -1471,18 +1471,19 @@ __attribute__((amdgpu_waves_per_eu(WRITE || (ALG == SJB_ALG_SMJ && GROUPED) ? 1 
     join_tail(acc, s_red, partials, p, direct);
 }
 
-// Count-path HJ join of one window on a resident grid (the host launches it
-// when the partition pass zeroed `direct` and load_view_small applies):
+// Count-path HJ / SMJ join of one window on a resident grid (the host
+// launches it when the partition pass zeroed `direct` and load_view_small
+// applies):
 // workgroup b joins partitions xcd_partition(b), xcd_partition(b + G), ...
 // (G a multiple of the 8 XCDs, so all of them on b's XCD), and loads the
 // next partition's run bounds while the current one's records are gathered,
 // so that global round trip leaves the per-partition chain.  Its
 // (count, C1, C2) over all of them is added to the window result once.
-template <bool C2>
+template <int ALG, bool C2>
 __global__ __launch_bounds__(kJoinThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_join_pipe(
     JoinIO io, const uint32_t *__restrict__ offs, int logF, int ncL, int ncR, uint32_t nL, int chunk, int bshift,
     sjb_window_result_t *__restrict__ direct) {
-    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<SJB_ALG_HJ, false, false>::value];
+    __shared__ __align__(16) uint8_t s_mem[JoinLdsBytes<ALG, false, false>::value];
     __shared__ uint32_t s_ws[kJoinThreads / kWave];
     __shared__ uint64_t s_red[3 * kJoinThreads / kWave];
     __shared__ unsigned long long s_base;
@@ -1496,10 +1497,10 @@ __global__ __launch_bounds__(kJoinThreads) __attribute__((amdgpu_waves_per_eu(6)
     acc.zero();
     for (; it < F; it += G) {
         __syncthreads();  // the previous partition's LDS is no longer read
-        const PartView v = view_small_from(dyn, a, b, ncL, ncR, 0u, nL, uint32_t(chunk), kHjCap, s_ws);
+        const PartView v =
+            view_small_from(dyn, a, b, ncL, ncR, 0u, nL, uint32_t(chunk), idcap_of<ALG, false>(), s_ws);
         if (it + G < F) view_bounds(offs, logF, xcd_partition(it + G, F), nq, a, b);
-        hj_join_part<C2, false, false>(io, v, bshift, *reinterpret_cast<HjLds<false> *>(s_mem), s_ws, &s_base,
-                                       acc);
+        join_part<ALG, C2, false, false, true>(io, v, bshift, s_mem, s_ws, &s_base, acc);
     }
     join_tail(acc, s_red, nullptr, 0, direct);
 }
@@ -1653,17 +1654,16 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     io.out_cap = out_cap;
     io.wrap = wrap;
     io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
-    if (ALG == SJB_ALG_HJ && p.payload == PAY_VAL && !grouped && p.direct && p.ncL + p.ncR <= kJoinThreads &&
-        p.ncL % kWave == 0) {
+    if (p.payload == PAY_VAL && !grouped && p.direct && p.ncL + p.ncR <= kJoinThreads && p.ncL % kWave == 0) {
         // resident grid: three workgroups per CU, a multiple of the 8 XCDs
         const int slots = c->n_cu > 0 ? 3 * c->n_cu : F;
         const int G = F <= slots ? F : std::max(slots & ~7, 8);
         if (c2)
-            k_join_pipe<true><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
-                                                                    uint32_t(p.nL), p.chunk, bshift, p.direct);
+            k_join_pipe<ALG, true><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
+                                                                         uint32_t(p.nL), p.chunk, bshift, p.direct);
         else
-            k_join_pipe<false><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
-                                                                     uint32_t(p.nL), p.chunk, bshift, p.direct);
+            k_join_pipe<ALG, false><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
+                                                                          uint32_t(p.nL), p.chunk, bshift, p.direct);
         SJB_HIP_TRY(hipGetLastError());
         return SJB_OK;
     }
