@@ -234,8 +234,8 @@ def run_window(a, np, torch, sjb):
     phases = {"partition": {"alg_bytes": 24 * 2 * W, "ms": float(part_ms.mean()),
                             "kernels": "partition pass (k_partition_pipe; SMJ: + key-range sample)"},
               "join": {"alg_bytes": 8 * 2 * W, "ms": float(join_ms.mean()),
-                       "kernels": "HJ: k_join_pipe (resident grid), SMJ: k_join; both sum into the window "
-                                  "result (no reduction kernel)"}}
+                       "kernels": "k_join_pipe (resident grid; sums into the window result, no reduction "
+                                  "kernel)"}}
     dom = max(phases, key=lambda k: phases[k]["ms"])
     for v in phases.values():
         v["achieved_GBps"] = v["alg_bytes"] / (v["ms"] / 1e3) / 1e9
