@@ -339,6 +339,38 @@ def test_full_size_window(ctx, torch_mod, orc, sjbmod):
     assert 0.9 * 2**20 < o.count < 1.1 * 2**20
 
 
+@pytest.mark.parametrize("alg", [HJ, SMJ])
+def test_resident_join_walk(ctx, torch_mod, orc, sjbmod, alg):
+    # 2^21 tuples per side: HJ 2^10 and SMJ 2^11 partitions, more than the
+    # resident count join's 768 workgroups, so each walks several partitions
+    # of its XCD; count + C1 only and count + C1 + C2.
+    n = 1 << 21
+    print(f"case resident walk alg={alg} n={n}", flush=True)
+    R = sjbmod.generate(n, sjbmod.DIST_UNIFORM, 1 << 23, 0.0, seed=41)
+    S = sjbmod.generate(n, sjbmod.DIST_UNIFORM, 1 << 23, 0.0, seed=42)
+    o, _ = orc.lazy_hj(R, S, 1 << 20)
+    st = run_count(ctx, torch_mod, alg, R, S, flags=0)
+    assert (st.result_count, st.checksum1) == o.triple()[:2]
+    assert st.fanout_log2 >= 10
+    st = run_count(ctx, torch_mod, alg, R, S)
+    assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+
+
+def test_resident_join_skew(ctx, torch_mod, orc, sjbmod):
+    # Zipf(0.5) keys over [0, 2^22), 2^21 tuples per side: partitions of
+    # uneven sizes (the hottest key has ~500 copies per side, ~2 matches per
+    # tuple) walked by the resident join.
+    n = 1 << 21
+    print(f"case resident skew n={n}", flush=True)
+    R = sjbmod.generate(n, sjbmod.DIST_ZIPF, 1 << 22, 0.5, seed=43)
+    S = sjbmod.generate(n, sjbmod.DIST_ZIPF, 1 << 22, 0.5, seed=44)
+    o, _ = orc.lazy_hj(R, S, 1 << 20)
+    for alg in (HJ, SMJ):
+        st = run_count(ctx, torch_mod, alg, R, S)
+        assert (st.result_count, st.checksum1, st.checksum2) == o.triple(), alg
+    assert o.count > n
+
+
 def test_window_above_2_22(torch_mod, orc, sjbmod):
     # 2^23 + 12345 / 2^23 - 777 tuples per side (keys uniform in [0, 2^25)):
     # HJ at fanout 2^12 with half-full tiles and SMJ at 2^12, 513 / 512
