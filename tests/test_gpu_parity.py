@@ -341,9 +341,10 @@ def test_full_size_window(ctx, torch_mod, orc, sjbmod):
 
 @pytest.mark.parametrize("alg", [HJ, SMJ])
 def test_resident_join_walk(ctx, torch_mod, orc, sjbmod, alg):
-    # 2^21 tuples per side: HJ 2^10 and SMJ 2^11 partitions, more than the
-    # resident count join's 768 workgroups, so each walks several partitions
-    # of its XCD; count + C1 only and count + C1 + C2.
+    # 2^21 tuples per side: HJ 2^10 (grouped 2^12) and SMJ 2^11 partitions,
+    # more than the resident count join's 768 workgroups, so each walks
+    # several partitions of its XCD; count + C1 only, count + C1 + C2 and
+    # grouped.
     n = 1 << 21
     print(f"case resident walk alg={alg} n={n}", flush=True)
     R = sjbmod.generate(n, sjbmod.DIST_UNIFORM, 1 << 23, 0.0, seed=41)
@@ -354,6 +355,8 @@ def test_resident_join_walk(ctx, torch_mod, orc, sjbmod, alg):
     assert st.fanout_log2 >= 10
     st = run_count(ctx, torch_mod, alg, R, S)
     assert (st.result_count, st.checksum1, st.checksum2) == o.triple()
+    st = run_count(ctx, torch_mod, alg, R, S, flags=F_GROUPED)
+    assert (st.result_count, st.checksum1) == o.triple()[:2]
 
 
 def test_resident_join_skew(ctx, torch_mod, orc, sjbmod):
@@ -368,6 +371,8 @@ def test_resident_join_skew(ctx, torch_mod, orc, sjbmod):
     for alg in (HJ, SMJ):
         st = run_count(ctx, torch_mod, alg, R, S)
         assert (st.result_count, st.checksum1, st.checksum2) == o.triple(), alg
+        st = run_count(ctx, torch_mod, alg, R, S, flags=F_GROUPED)
+        assert (st.result_count, st.checksum1) == o.triple()[:2], alg
     assert o.count > n
 
 
