@@ -48,6 +48,10 @@ constexpr int kHjProbe = SJB_HJ_PROBE;                  // probe records staged 
 #ifndef SJB_BATCH_HJ_WAVES
 #define SJB_BATCH_HJ_WAVES 6
 #endif
+// workgroups per CU launched for the resident count join (three fit)
+#ifndef SJB_JOIN_PIPE_WG_PER_CU
+#define SJB_JOIN_PIPE_WG_PER_CU 3
+#endif
 #ifndef SJB_BATCH_SMJ_WAVES
 #define SJB_BATCH_SMJ_WAVES 6
 #endif
@@ -1656,7 +1660,7 @@ static int launch_join_alg(sjb_ctx *c, const WindowPlan &p, const sjb_tuple_t *L
     io.cursor = reinterpret_cast<unsigned long long *>(c->d_cursor);
     if (p.payload == PAY_VAL && !grouped && p.direct && p.ncL + p.ncR <= kJoinThreads && p.ncL % kWave == 0) {
         // resident grid: three workgroups per CU, a multiple of the 8 XCDs
-        const int slots = c->n_cu > 0 ? 3 * c->n_cu : F;
+        const int slots = c->n_cu > 0 ? SJB_JOIN_PIPE_WG_PER_CU * c->n_cu : F;
         const int G = F <= slots ? F : std::max(slots & ~7, 8);
         if (c2)
             k_join_pipe<ALG, true><<<G, kJoinThreads, dynb, c->stream>>>(io, p.offs, p.logF, p.ncL, p.ncR,
